@@ -1,0 +1,10 @@
+"""edward_amd -- MI355X-native engine behind Edward's ``ed.HMC`` surface.
+
+Import as ``import edward_amd as ed``.  The public names mirror the subset of
+Edward 1.3.5's API (``edward/__init__.py:32-90``) that an HMC program over a
+generalised linear model uses: ``ed.HMC``, ``ed.models.{Normal, Bernoulli,
+Empirical}``, ``ed.dot``, ``ed.set_seed``, ``ed.get_session``,
+``ed.placeholder``.  The sampler's arithmetic runs in ``libedhmc.so``.
+"""
+
+from edward_amd.version import __version__  # noqa: F401

@@ -1,0 +1,235 @@
+"""Match an Edward model to a device engine.
+
+The reference evaluates the log joint by copying the model graph at the
+proposed sample (``hmc.py:161-192`` via ``util/random_variables.py:139-445``).
+Here the model DAG is matched once to a kernel family:
+
+  * GLM: one observed ``Bernoulli(logits=E)`` or ``Normal(loc=E, scale=s)``
+    with E a sum of ``ed.dot(X, w)`` terms, bare latents broadcast over rows
+    (a bias, or the mean of ``Normal(loc=mu, sample_shape=n)``) and constant
+    multiples of these; every latent ``Normal(loc, scale)`` with constant
+    parameters.  The latents are concatenated into one vector and the
+    design matrix gets one column per coordinate (ones for broadcast terms).
+  * Hierarchical: ``Bernoulli(logits=ed.gather(a, group) + ed.dot(X, w))``
+    with ``a ~ Normal(mu, exp(log_sigma))``, ``mu, log_sigma ~ N(0, 1)``,
+    ``w ~ N(0, I)``.
+
+Anything else raises ``NotImplementedError``.
+"""
+
+import numpy as np
+import torch
+
+from edward_amd import models as M
+
+
+def _array(v):
+  if isinstance(v, torch.Tensor):
+    return v.detach().cpu().numpy()
+  return np.asarray(v)
+
+
+def _resolve(node, data):
+  """Constant value of a data node (placeholder bound in data, or array)."""
+  if isinstance(node, M.Placeholder):
+    if node not in data:
+      raise ValueError('placeholder %s is not bound in data' % node.name)
+    return _array(data[node]).astype(np.float64)
+  if isinstance(node, (M.RandomVariable, M.Node)):
+    return None
+  return _array(node).astype(np.float64)
+
+
+class _Term(object):
+  def __init__(self, rv, cols):
+    self.rv = rv        # latent RandomVariable
+    self.cols = cols    # [N, k] coefficient block (or callable(N) -> block)
+
+
+def _linear_terms(E, data, latents):
+  """Flatten E into (latent, coefficient block) terms."""
+  if isinstance(E, M.Add):
+    return _linear_terms(E.a, data, latents) + _linear_terms(E.b, data, latents)
+  if isinstance(E, M.Mul):
+    ca, cb = _resolve(E.a, data), _resolve(E.b, data)
+    if ca is not None and cb is None:
+      const, other = ca, E.b
+    elif cb is not None and ca is None:
+      const, other = cb, E.a
+    else:
+      raise NotImplementedError('only constant * latent-expression products are supported')
+    if np.ndim(const) != 0:
+      raise NotImplementedError('only scalar multipliers are supported')
+    terms = _linear_terms(other, data, latents)
+    for t in terms:
+      t.cols = _scaled(t.cols, float(const))
+    return terms
+  if isinstance(E, M.Dot):
+    x, y = E.x, E.y
+    if isinstance(y, M.RandomVariable) and y in latents:
+      mat = _resolve(x, data)
+      if mat is None or mat.ndim != 2:
+        raise NotImplementedError('ed.dot(X, w) needs a data matrix X')
+      return [_Term(y, mat)]
+    if isinstance(x, M.RandomVariable) and x in latents:
+      mat = _resolve(y, data)
+      if mat is None or mat.ndim != 2:
+        raise NotImplementedError('ed.dot(w, X) needs a data matrix X')
+      return [_Term(x, mat.T)]
+    raise NotImplementedError('ed.dot must pair a data matrix with a latent vector')
+  if isinstance(E, M.RandomVariable):
+    if E not in latents:
+      raise NotImplementedError('%s is neither a latent nor data' % E.name)
+    k = int(np.prod(E.shape)) if E.shape else 1
+    if k != 1:
+      raise NotImplementedError('only scalar latents can be broadcast over rows')
+    return [_Term(E, 'ones')]
+  c = _resolve(E, data)
+  if c is not None:
+    if np.any(c != 0):
+      raise NotImplementedError('constant offsets in the linear predictor are not supported')
+    return []
+  raise NotImplementedError('unsupported node %r in the linear predictor' % (E,))
+
+
+def _scaled(cols, s):
+  if isinstance(cols, str):
+    return ('ones', s)
+  if isinstance(cols, tuple):
+    return (cols[0], cols[1] * s)
+  return cols * s
+
+
+def _block(cols, N):
+  if isinstance(cols, str):
+    return np.ones((N, 1))
+  if isinstance(cols, tuple):
+    return np.full((N, 1), cols[1])
+  if cols.shape[0] != N:
+    raise ValueError('design block has %d rows, observations have %d' % (cols.shape[0], N))
+  return cols
+
+
+class Latent(object):
+  def __init__(self, rv, qz, k, offset, n_chains):
+    self.rv, self.qz, self.k, self.offset, self.n_chains = rv, qz, k, offset, n_chains
+
+
+class Program(object):
+  """A compiled model: the engine plus the map between the latent vector and
+  the Empirical stores."""
+
+  def __init__(self, engine, latents, n_rows, n_chains, dim):
+    self.engine = engine
+    self.latents = latents
+    self.n_rows = n_rows
+    self.n_chains = n_chains
+    self.dim = dim
+
+  def gather(self, t):
+    """``params[t]`` of every latent as one [C, dim] float32 block."""
+    parts = [l.qz.params[t].reshape(self.n_chains, l.k).to(torch.float32)
+             for l in self.latents]
+    return torch.cat(parts, dim=1).contiguous()
+
+  def scatter(self, t, row):
+    for l in self.latents:
+      dst = l.qz.params[t]
+      dst.copy_(row[:, l.offset:l.offset + l.k].reshape(dst.shape))
+
+
+def _chains_of(rv, qz):
+  ev = tuple(qz.event_shape)
+  ks = tuple(rv.shape)
+  if ev == ks:
+    return 1
+  if len(ev) == len(ks) + 1 and ev[1:] == ks:
+    return ev[0]
+  raise TypeError('Empirical of shape %s does not fit latent of shape %s' % (ev, ks))
+
+
+def _normal_prior(rv, k):
+  if not isinstance(rv, M.Normal):
+    raise NotImplementedError('latent %s must be a Normal' % rv.name)
+  loc, scale = _resolve(rv.loc, {}), _resolve(rv.scale, {})
+  if loc is None or scale is None:
+    return None
+  shape = rv.shape if rv.shape else ()
+  loc = np.broadcast_to(loc, shape).reshape(-1) if shape else np.full(1, float(loc))
+  scale = np.broadcast_to(scale, shape).reshape(-1) if shape else np.full(1, float(scale))
+  if loc.size != k:
+    loc = np.resize(loc, k)
+    scale = np.resize(scale, k)
+  if np.any(scale <= 0):
+    raise ValueError('prior scale of %s must be positive' % rv.name)
+  return loc, scale
+
+
+def compile_model(latent_vars, data, precision='fp32'):
+  from edward_amd.engine import GLMEngine
+  observed = [k for k in data if isinstance(k, M.RandomVariable)]
+  if len(observed) != 1:
+    raise NotImplementedError('exactly one observed random variable is supported, got %d'
+                              % len(observed))
+  obs = observed[0]
+  yv = _array(data[obs]).astype(np.float64).reshape(-1)
+  latents = list(latent_vars.keys())
+  if isinstance(obs, M.Bernoulli) and isinstance(obs.logits, M.Add) and \
+          _has_gather(obs.logits):
+    from edward_amd import hier
+    return hier.compile_hier(latent_vars, data, obs, yv)
+  if isinstance(obs, M.Bernoulli):
+    if obs.logits is None:
+      raise NotImplementedError('Bernoulli(probs=...) likelihoods are not supported; '
+                                'use logits')
+    E, likelihood, lik_scale = obs.logits, 'bernoulli', 1.0
+  elif isinstance(obs, M.Normal):
+    s = _resolve(obs.scale, data)
+    if s is None or np.ndim(s) > 1 or np.any(s != np.ravel(s)[0]):
+      raise NotImplementedError('Normal likelihood needs one constant scale')
+    E, likelihood, lik_scale = obs.loc, 'normal', float(np.ravel(s)[0])
+  else:
+    raise NotImplementedError('unsupported likelihood %s' % type(obs).__name__)
+  terms = _linear_terms(E, data, set(latents))
+  N = yv.size
+  # Latent vector layout follows latent_vars' order.
+  blocks, locs, scales, lat = [], [], [], []
+  off = 0
+  n_chains = None
+  for rv in latents:
+    qz = latent_vars[rv]
+    k = int(np.prod(rv.shape)) if rv.shape else 1
+    C = _chains_of(rv, qz)
+    if n_chains is None:
+      n_chains = C
+    elif C != n_chains:
+      raise ValueError('all Empirical stores must hold the same number of chains')
+    pr = _normal_prior(rv, k)
+    if pr is None:
+      raise NotImplementedError('prior parameters of %s must be constants' % rv.name)
+    locs.append(pr[0])
+    scales.append(pr[1])
+    blk = np.zeros((N, k))
+    for t in terms:
+      if t.rv is rv:
+        b = _block(t.cols, N)
+        if b.shape[1] != k:
+          raise ValueError('ed.dot operand of %d columns for latent of size %d'
+                           % (b.shape[1], k))
+        blk = blk + b
+    blocks.append(blk)
+    lat.append(Latent(rv, qz, k, off, C))
+    off += k
+  X = np.concatenate(blocks, axis=1).astype(np.float32)
+  T = int(min(l.qz.params.shape[0] for l in lat))
+  eng = GLMEngine(X, yv.astype(np.float32), np.concatenate(locs), np.concatenate(scales),
+                  likelihood=likelihood, lik_scale=lik_scale, precision=precision)
+  return Program(eng, lat, T, n_chains, off)
+
+
+def _has_gather(E):
+  if isinstance(E, M.Gather):
+    return True
+  if isinstance(E, M.Add):
+    return _has_gather(E.a) or _has_gather(E.b)
+  return False

@@ -1,0 +1,113 @@
+// Shared device helpers for the MI355X (gfx950) HMC engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace edhmc {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8_t;
+typedef __attribute__((ext_vector_type(16))) float f32x16_t;
+typedef __attribute__((ext_vector_type(4))) float f32x4_t;
+
+// bf16 is carried as its raw 16-bit pattern.
+typedef uint16_t bf16_t;
+
+constexpr double kLog2Pi = 1.8378770664093454835606594728112;
+
+// Round-to-nearest-even float -> bf16 (NaN kept quiet).
+__device__ __host__ inline bf16_t f32_to_bf16(float f) {
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16_t)0x7fc0;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16_t)(u >> 16);
+}
+
+__device__ __host__ inline float bf16_to_f32(bf16_t h) {
+  return __builtin_bit_cast(float, (uint32_t)h << 16);
+}
+
+template <typename T> struct ElemOps;
+template <> struct ElemOps<float> {
+  static constexpr int kBytes = 4;
+  __device__ static inline float from_f32(float v) { return v; }
+};
+template <> struct ElemOps<bf16_t> {
+  static constexpr int kBytes = 2;
+  __device__ static inline bf16_t from_f32(float v) { return f32_to_bf16(v); }
+};
+
+// ---------------------------------------------------------------------------
+// Block reductions in FP64 with a fixed combination order (deterministic).
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum_f64(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// All threads of the block must call; returns the block total to every thread.
+template <int NT>
+__device__ inline double block_sum_f64(double v, double* sm /* >= NT/64 */) {
+  v = wave_sum_f64(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sm[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) t += sm[w];
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// Likelihood terms of the generalised-linear models.
+//   Bernoulli(logits=eta):  ll = y eta - softplus(eta),  dll/deta = y - sigmoid(eta)
+//   Normal(loc=eta, s):     ll = -(y-eta)^2/(2 s^2) - log s - log(2 pi)/2,
+//                           dll/deta = (y - eta)/s^2
+// ---------------------------------------------------------------------------
+enum { LIK_BERNOULLI = 0, LIK_NORMAL = 1 };
+
+struct LikParams {
+  int lik;
+  float inv_s2;    // 1/s^2 (normal)
+  double ll_const; // -log s - log(2 pi)/2 (normal)
+};
+
+__device__ inline float glm_dll(const LikParams& p, float eta, float y) {
+  if (p.lik == LIK_BERNOULLI) {
+    float e = __expf(-fabsf(eta));
+    float inv = __frcp_rn(1.0f + e);
+    float sig = eta >= 0.0f ? inv : e * inv;
+    return y - sig;
+  }
+  return (y - eta) * p.inv_s2;
+}
+
+__device__ inline double glm_ll(const LikParams& p, float eta, float y) {
+  if (p.lik == LIK_BERNOULLI) {
+    float e = __expf(-fabsf(eta));
+    float sp = fmaxf(eta, 0.0f) + log1pf(e);
+    return (double)(y * eta) - (double)sp;
+  }
+  float d = y - eta;
+  return -0.5 * (double)(d * d) * (double)p.inv_s2 + p.ll_const;
+}
+
+__device__ inline void glm_both(const LikParams& p, float eta, float y,
+                                float& dll, double& ll) {
+  if (p.lik == LIK_BERNOULLI) {
+    float e = __expf(-fabsf(eta));
+    float inv = __frcp_rn(1.0f + e);
+    float sig = eta >= 0.0f ? inv : e * inv;
+    dll = y - sig;
+    float sp = fmaxf(eta, 0.0f) + log1pf(e);
+    ll = (double)(y * eta) - (double)sp;
+  } else {
+    float d = y - eta;
+    dll = d * p.inv_s2;
+    ll = -0.5 * (double)(d * d) * (double)p.inv_s2 + p.ll_const;
+  }
+}
+
+}  // namespace edhmc

@@ -1,0 +1,134 @@
+// Library context shared by the GLM engine, the surrogate and the
+// hierarchical engine.  Host-side only (never passed to kernels).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/edhmc.h"
+
+namespace edhmc {
+
+struct Surrogate;   // mlp.hip
+struct HierModel;   // hier.hip
+
+// A device allocation owned by the context.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+}  // namespace edhmc
+
+struct edhmc_ctx {
+  int device = 0;
+  std::string err;
+
+  // ---- generalised linear model ----
+  bool bound = false;
+  int cdt = EDHMC_F32;   // compute dtype of X / positions in the GEMMs
+  int lik = EDHMC_LIK_BERNOULLI;
+  float lik_scale = 1.0f;
+  int64_t N = 0, N_pad = 0;
+  int D = 0, D_pad = 0;
+  edhmc::DevBuf Xr, Xc, y, ploc, pscale;
+
+  // ---- chains ----
+  int C = 0, C_pad = 0;
+  bool state_valid = false;
+  edhmc::DevBuf z, r, g, z0, g0, z_lp, logp, K0, flag, ndiv;
+  // ---- gradient scratch ----
+  edhmc::DevBuf Rt, Gpart, llpart;
+  int ksplit = 1;
+  int n_tiles1 = 0;      // GEMM1 column tiles (= llpart leading dim)
+  // ---- moments ----
+  bool mom_on = false;
+  int64_t mom_count = 0;
+  edhmc::DevBuf mom_mean, mom_m2;
+  // ---- misc scratch ----
+  edhmc::DevBuf tmp_count;
+
+  edhmc::Surrogate* sur = nullptr;
+  edhmc::HierModel* hier = nullptr;
+
+  ~edhmc_ctx();
+};
+
+namespace edhmc {
+
+inline int set_err(edhmc_ctx* ctx, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  return code;
+}
+
+inline int dev_alloc(edhmc_ctx* ctx, DevBuf& b, size_t bytes, bool zero = true) {
+  if (b.p && b.bytes == bytes) {
+    if (zero && hipMemset(b.p, 0, bytes) != hipSuccess)
+      return set_err(ctx, EDHMC_EHIP, "hipMemset failed");
+    return EDHMC_OK;
+  }
+  b.release();
+  if (bytes == 0) return EDHMC_OK;
+  if (hipMalloc(&b.p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    b.p = nullptr;
+    return set_err(ctx, EDHMC_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+  }
+  b.bytes = bytes;
+  if (zero && hipMemset(b.p, 0, bytes) != hipSuccess)
+    return set_err(ctx, EDHMC_EHIP, "hipMemset failed");
+  return EDHMC_OK;
+}
+
+inline int check_launch(edhmc_ctx* ctx, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return set_err(ctx, EDHMC_EHIP, "%s: %s", what, hipGetErrorString(e));
+  return EDHMC_OK;
+}
+
+inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// RAII device guard: make ctx->device current for the call.
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+#define EDHMC_TRY(expr)                 \
+  do {                                  \
+    int _rc = (expr);                   \
+    if (_rc != EDHMC_OK) return _rc;    \
+  } while (0)
+
+// Implemented in the other translation units.
+void surrogate_destroy(Surrogate* s);
+void hier_destroy(HierModel* h);
+
+// GLM pieces reused by the surrogate transition (edhmc_glm.hip).
+int glm_forward_ll(edhmc_ctx* ctx, hipStream_t st);   // llpart at z_lp (split)
+int glm_write_lp(edhmc_ctx* ctx, hipStream_t st);     // z -> z_lp planes
+
+}  // namespace edhmc

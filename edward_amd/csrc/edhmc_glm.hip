@@ -1,0 +1,920 @@
+// HMC engine for generalised linear models on MI355X (gfx950).
+//
+// One gradient evaluation of the log joint at the C chains' positions Z [C, D]
+// (the work of hmc.py:199/206 tf.gradients over hmc.py:161-192) is
+//   GEMM1  eta^T [C, N] = Z . X^T         (MFMA; epilogue: residual
+//          R^T = dll/deta written chain-major, and the FP64 log-likelihood
+//          partial sums per (chain, row tile) when the ratio needs them)
+//   GEMM2  G [C, D] = R^T . X              (MFMA, split-K over rows)
+//   chain  sum the split-K partials, add the prior gradient, then the
+//          leapfrog kick/drift -- or, after the last step, the Metropolis
+//          test of hmc.py:100-111 and the sample write of hmc.py:121-129.
+// X is stored twice, row-major Xr [N, D] (GEMM1's B) and column-major
+// Xc [D, N] (GEMM2's B), so every MFMA operand is K-contiguous.
+#include <algorithm>
+#include <cmath>
+
+#include "common.h"
+#include "ctx.h"
+#include "gemm_nt.h"
+#include "philox.h"
+
+namespace edhmc {
+
+using Cfg1 = TileCfg<2, 2, 2, 2>;   // 128 x 128, GEMM1 and wide GEMM2
+using Cfg2S = TileCfg<1, 1, 4, 1>;  // 128 x 32, GEMM2 for D_pad % 128 != 0
+constexpr int kChainThreads = 256;
+
+// ---------------------------------------------------------------------------
+// GEMM1 epilogue: residuals and FP64 log-likelihood partials.
+// Accumulator element (m, n) = eta of chain m at data row n.
+// ---------------------------------------------------------------------------
+template <typename T, bool WR, bool LL>
+struct EpiGLM {
+  const float* y;
+  int64_t N;
+  LikParams lp;
+  T* Rt;              // [C_pad, N_pad]
+  int64_t ldr;
+  double* llpart;     // [C_pad, n_tiles]
+  int64_t ll_ld;
+
+  template <typename Cfg>
+  __device__ void apply(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p) const {
+    constexpr int WM = Cfg::kWM, WN = Cfg::kWN;
+    double lls[WM][16];
+    if (LL) {
+#pragma unroll
+      for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) lls[i][r] = 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      const int64_t n = p.n0 + 32 * j + (p.lane & 31);
+      const bool valid = n < N;
+      const float yv = valid ? y[n] : 0.0f;
+#pragma unroll
+      for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const float eta = acc[i][j][r];
+          float dll;
+          double ll;
+          if (LL) {
+            glm_both(lp, eta, yv, dll, ll);
+            lls[i][r] += valid ? ll : 0.0;
+          } else {
+            dll = glm_dll(lp, eta, yv);
+          }
+          if (WR) {
+            const int64_t m = p.m0 + 32 * i + acc_row(r, p.lane);
+            Rt[m * ldr + n] = ElemOps<T>::from_f32(valid ? dll : 0.0f);
+          }
+        }
+    }
+    if (LL) {
+      constexpr int BM = Cfg::BM;
+      double* sm = reinterpret_cast<double*>(p.smem);
+#pragma unroll
+      for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          double v = lls[i][r];
+#pragma unroll
+          for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+          if ((p.lane & 31) == 0)
+            sm[p.wn * BM + p.wm * 32 * WM + 32 * i + acc_row(r, p.lane)] = v;
+        }
+      __syncthreads();
+      for (int t = threadIdx.x; t < BM; t += Cfg::kThreads) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < Cfg::kWavesN; ++w) s += sm[w * BM + t];
+        llpart[(int64_t)(p.tm * BM + t) * ll_ld + p.tn] = s;
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Data layout: X [N, D] (fp32 or bf16 in) -> Xr [N_pad, D_pad], Xc [D_pad, N_pad]
+// in the compute type; counts non-finite inputs.
+// ---------------------------------------------------------------------------
+template <typename Tin, typename T>
+__global__ void __launch_bounds__(256)
+layout_x_kernel(const Tin* __restrict__ X, int64_t N, int D, T* __restrict__ Xr,
+                T* __restrict__ Xc, int64_t N_pad, int D_pad, int* nonfinite) {
+  __shared__ float tile[32][33];
+  const int64_t n0 = (int64_t)blockIdx.y * 32;
+  const int d0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  int bad = 0;
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t n = n0 + k;
+    const int d = d0 + tx;
+    float v = 0.0f;
+    if (n < N && d < D) {
+      if constexpr (sizeof(Tin) == 2) v = bf16_to_f32(((const bf16_t*)X)[n * D + d]);
+      else v = ((const float*)X)[n * D + d];
+      if (!isfinite(v)) { bad = 1; v = 0.0f; }
+    }
+    tile[k][tx] = v;
+    Xr[n * D_pad + d] = ElemOps<T>::from_f32(v);
+  }
+  if (bad) atomicAdd(nonfinite, 1);
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int d = d0 + k;
+    const int64_t n = n0 + tx;
+    Xc[(int64_t)d * N_pad + n] = ElemOps<T>::from_f32(tile[tx][k]);
+  }
+}
+
+__global__ void pad_copy_kernel(const float* __restrict__ src, int64_t n, float* __restrict__ dst,
+                                int64_t n_pad, float fill, int* nonfinite) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_pad) return;
+  float v = fill;
+  if (i < n) {
+    v = src[i];
+    if (!isfinite(v)) atomicAdd(nonfinite, 1);
+  }
+  dst[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Chain kernels (one workgroup per chain).
+// ---------------------------------------------------------------------------
+struct ChainArgs {
+  int C, D, D_pad, C_pad;
+  int bf16;                 // z_lp holds hi/lo bf16 planes (else fp32)
+  float* z; float* r; float* g; float* z0; float* g0;
+  void* z_lp;
+  double* logp; double* K0;
+  const float* ploc; const float* pscale;
+  float half, full;         // fl32(eps/2), fl32(eps)
+  const float* r_fed; const double* u_fed;
+  uint64_t seed, counter;
+  const float* Gpart; int ksplit;
+  const double* llpart; int ll_ld; int n_ll;
+  float* sample_out; int32_t* accepted_out; int32_t* n_accept; int32_t* ndiv;
+  double* mom_mean; double* mom_m2; int64_t mom_n;   // mom_n = count after update
+  float* grad_out; double* logp_out;
+  const float* W_in;        // set_state input [C, D]
+};
+
+__device__ inline void write_lp(const ChainArgs& a, int c, int d, float zv) {
+  const int64_t idx = (int64_t)c * a.D_pad + d;
+  if (a.bf16) {
+    bf16_t* p = (bf16_t*)a.z_lp;
+    bf16_t hi = f32_to_bf16(zv);
+    bf16_t lo = f32_to_bf16(zv - bf16_to_f32(hi));
+    p[idx] = hi;
+    p[idx + (int64_t)a.C_pad * a.D_pad] = lo;
+  } else {
+    ((float*)a.z_lp)[idx] = zv;
+  }
+}
+
+__device__ inline float prior_grad(const ChainArgs& a, int d, float zv) {
+  const float s = a.pscale[d];
+  return -(zv - a.ploc[d]) / (s * s);
+}
+
+__device__ inline double prior_logp(const ChainArgs& a, int d, float zv) {
+  const double s = a.pscale[d];
+  const double t = ((double)zv - (double)a.ploc[d]) / s;
+  return -0.5 * t * t - log(s) - 0.5 * kLog2Pi;
+}
+
+// Momentum draw, K0 = |r0|^2 / 2, save (z0, g0), first half kick and drift.
+__global__ void __launch_bounds__(kChainThreads) chain_begin_kernel(ChainArgs a) {
+  __shared__ double red[kChainThreads / 64];
+  const int c = blockIdx.x;
+  double k = 0.0;
+  const int nb = a.D_pad / 4;
+  for (int j = threadIdx.x; j < nb; j += kChainThreads) {
+    f32x4_t rv;
+    if (a.r_fed) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int d = 4 * j + e;
+        rv[e] = d < a.D ? a.r_fed[(int64_t)c * a.D + d] : 0.0f;
+      }
+    } else {
+      rv = momentum4(a.seed, a.counter, (uint32_t)j, (uint32_t)c);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int d = 4 * j + e;
+      if (d >= a.D) continue;
+      const int64_t idx = (int64_t)c * a.D_pad + d;
+      float r = rv[e];
+      k += (double)r * (double)r;
+      const float zv = a.z[idx], gv = a.g[idx];
+      a.z0[idx] = zv;
+      a.g0[idx] = gv;
+      r = r + a.half * gv;
+      const float zn = zv + a.full * r;
+      a.z[idx] = zn;
+      a.r[idx] = r;
+      write_lp(a, c, d, zn);
+    }
+  }
+  k = block_sum_f64<kChainThreads>(k, red);
+  if (threadIdx.x == 0) a.K0[c] = 0.5 * k;
+}
+
+__device__ inline float gsum(const ChainArgs& a, int c, int d) {
+  const int64_t stride = (int64_t)a.C_pad * a.D_pad;
+  const float* p = a.Gpart + (int64_t)c * a.D_pad + d;
+  float s = 0.0f;
+  for (int q = 0; q < a.ksplit; ++q) s += p[q * stride];
+  return s;
+}
+
+__device__ inline double ll_total(const ChainArgs& a, int c, double* red) {
+  double s = 0.0;
+  const double* p = a.llpart + (int64_t)c * a.ll_ld;
+  for (int t = threadIdx.x; t < a.n_ll; t += kChainThreads) s += p[t];
+  return block_sum_f64<kChainThreads>(s, red);
+}
+
+// Middle of the trajectory: gradient at the new z, r += eps/2 g twice
+// (end of this step, start of the next), drift.
+__global__ void __launch_bounds__(kChainThreads) chain_mid_kernel(ChainArgs a) {
+  const int c = blockIdx.x;
+  for (int d = threadIdx.x; d < a.D; d += kChainThreads) {
+    const int64_t idx = (int64_t)c * a.D_pad + d;
+    const float zv = a.z[idx];
+    const float gv = gsum(a, c, d) + prior_grad(a, d, zv);
+    a.g[idx] = gv;
+    float r = a.r[idx];
+    r = r + a.half * gv;
+    r = r + a.half * gv;
+    const float zn = zv + a.full * r;
+    a.r[idx] = r;
+    a.z[idx] = zn;
+    write_lp(a, c, d, zn);
+  }
+}
+
+// Shared end of trajectory: last half kick, ratio, accept, select, store.
+// `g_at` supplies the gradient at the end point (exact or surrogate).
+template <typename GradAt>
+__device__ inline void chain_accept(const ChainArgs& a, GradAt g_at) {
+  __shared__ double red[kChainThreads / 64];
+  __shared__ int acc_flag;
+  const int c = blockIdx.x;
+  double kl = 0.0, plp = 0.0;
+  int bad = 0;
+  for (int d = threadIdx.x; d < a.D; d += kChainThreads) {
+    const int64_t idx = (int64_t)c * a.D_pad + d;
+    const float zv = a.z[idx];
+    const float gv = g_at(c, d, idx, zv);
+    a.g[idx] = gv;
+    const float r = a.r[idx] + a.half * gv;
+    a.r[idx] = r;
+    kl += (double)r * (double)r;
+    plp += prior_logp(a, d, zv);
+    if (!isfinite(zv) || !isfinite(gv)) bad = 1;
+  }
+  kl = block_sum_f64<kChainThreads>(kl, red);
+  plp = block_sum_f64<kChainThreads>(plp, red);
+  const double ll = ll_total(a, c, red);
+  const double nbad = block_sum_f64<kChainThreads>((double)bad, red);
+  if (threadIdx.x == 0) {
+    const double logpL = ll + plp;
+    const double ratio = a.K0[c] - 0.5 * kl + logpL - a.logp[c];
+    const double u = a.u_fed ? a.u_fed[c] : accept_uniform(a.seed, a.counter, (uint32_t)c);
+    // hmc.py:109: accept = log(u) < ratio (NaN never accepts).
+    const int acc = (log(u) < ratio) && nbad == 0.0;
+    acc_flag = acc;
+    if (acc) a.logp[c] = logpL;
+    if (a.accepted_out) a.accepted_out[c] = acc;
+    if (a.n_accept) a.n_accept[c] += acc;
+    if (a.ndiv && (nbad != 0.0 || !(ratio == ratio))) a.ndiv[c] += 1;
+  }
+  __syncthreads();
+  const int acc = acc_flag;
+  for (int d = threadIdx.x; d < a.D; d += kChainThreads) {
+    const int64_t idx = (int64_t)c * a.D_pad + d;
+    float zv;
+    if (acc) {
+      zv = a.z[idx];
+    } else {
+      zv = a.z0[idx];
+      a.z[idx] = zv;
+      a.g[idx] = a.g0[idx];
+    }
+    if (a.sample_out) a.sample_out[(int64_t)c * a.D + d] = zv;
+    if (a.mom_mean) {
+      const double x = zv;
+      const double mo = a.mom_mean[idx];
+      const double mn = mo + (x - mo) / (double)a.mom_n;
+      a.mom_mean[idx] = mn;
+      a.mom_m2[idx] += (x - mo) * (x - mn);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kChainThreads) chain_last_kernel(ChainArgs a) {
+  chain_accept(a, [&](int c, int d, int64_t, float zv) {
+    return gsum(a, c, d) + prior_grad(a, d, zv);
+  });
+}
+
+// Evaluation at the current z: gradient + log joint into the state (and
+// optional outputs).
+__global__ void __launch_bounds__(kChainThreads) chain_eval_kernel(ChainArgs a) {
+  __shared__ double red[kChainThreads / 64];
+  const int c = blockIdx.x;
+  double plp = 0.0;
+  for (int d = threadIdx.x; d < a.D; d += kChainThreads) {
+    const int64_t idx = (int64_t)c * a.D_pad + d;
+    const float zv = a.z[idx];
+    const float gv = gsum(a, c, d) + prior_grad(a, d, zv);
+    a.g[idx] = gv;
+    if (a.grad_out) a.grad_out[(int64_t)c * a.D + d] = gv;
+    plp += prior_logp(a, d, zv);
+  }
+  plp = block_sum_f64<kChainThreads>(plp, red);
+  const double ll = ll_total(a, c, red);
+  if (threadIdx.x == 0) {
+    a.logp[c] = ll + plp;
+    if (a.logp_out) a.logp_out[c] = ll + plp;
+  }
+}
+
+// Load positions W [C, D] into z and z_lp; count non-finite entries.
+__global__ void __launch_bounds__(kChainThreads) chain_load_kernel(ChainArgs a, int* nonfinite) {
+  const int c = blockIdx.x;
+  int bad = 0;
+  for (int d = threadIdx.x; d < a.D; d += kChainThreads) {
+    const int64_t idx = (int64_t)c * a.D_pad + d;
+    float v = a.W_in[(int64_t)c * a.D + d];
+    if (!isfinite(v)) bad = 1;
+    a.z[idx] = v;
+    write_lp(a, c, d, v);
+  }
+  if (bad) atomicAdd(nonfinite, 1);
+}
+
+// Rewrite z_lp from z for every chain.
+__global__ void __launch_bounds__(kChainThreads) chain_lp_kernel(ChainArgs a) {
+  const int c = blockIdx.x;
+  for (int d = threadIdx.x; d < a.D; d += kChainThreads)
+    write_lp(a, c, d, a.z[(int64_t)c * a.D_pad + d]);
+}
+
+__global__ void copy_out_kernel(const float* __restrict__ src, int C, int D, int D_pad,
+                                float* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)C * D) return;
+  const int c = (int)(i / D), d = (int)(i - (int64_t)c * D);
+  dst[i] = src[(int64_t)c * D_pad + d];
+}
+
+__global__ void copy_out_f64_kernel(const double* __restrict__ src, int C, int D, int D_pad,
+                                    double* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)C * D) return;
+  const int c = (int)(i / D), d = (int)(i - (int64_t)c * D);
+  dst[i] = src[(int64_t)c * D_pad + d];
+}
+
+__global__ void rng_momenta_kernel(int C, int D, uint64_t seed, uint64_t counter, float* out) {
+  const int c = blockIdx.x;
+  const int nb = (D + 3) / 4;
+  for (int j = threadIdx.x; j < nb; j += blockDim.x) {
+    f32x4_t v = momentum4(seed, counter, (uint32_t)j, (uint32_t)c);
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if (4 * j + e < D) out[(int64_t)c * D + 4 * j + e] = v[e];
+  }
+}
+
+__global__ void rng_uniforms_kernel(int C, uint64_t seed, uint64_t counter, double* out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < C) out[c] = accept_uniform(seed, counter, (uint32_t)c);
+}
+
+// Autocovariance for ESS: thread per (chain, coordinate); lags in blocks of 16
+// with a register window.
+__global__ void __launch_bounds__(256)
+autocov_kernel(const float* __restrict__ s, int64_t T, int C, int D, int max_lag,
+               double* __restrict__ acov, double* __restrict__ cmean) {
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (int64_t)C * D) return;
+  const int c = (int)(id / D), d = (int)(id - (int64_t)c * D);
+  const int64_t stride = (int64_t)C * D;
+  const float* x = s + (int64_t)c * D + d;
+  double m = 0.0;
+  for (int64_t i = 0; i < T; ++i) m += x[i * stride];
+  m /= (double)T;
+  if (cmean) cmean[id] = m;
+  for (int l0 = 0; l0 <= max_lag; l0 += 16) {
+    double acc[16];
+    float win[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      acc[k] = 0.0;
+      const int64_t j = l0 + k;
+      win[k] = j < T ? x[j * stride] : 0.0f;
+    }
+    for (int64_t i = 0; i + l0 < T; ++i) {
+      const double xi = (double)x[i * stride] - m;
+#pragma unroll
+      for (int k = 0; k < 16; ++k)
+        if (i + l0 + k < T) acc[k] += xi * ((double)win[k] - m);
+#pragma unroll
+      for (int k = 0; k < 15; ++k) win[k] = win[k + 1];
+      const int64_t j = i + l0 + 16;
+      win[15] = j < T ? x[j * stride] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (l0 + k <= max_lag) atomicAdd(&acov[(int64_t)(l0 + k) * D + d], acc[k] / (double)T);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side orchestration
+// ---------------------------------------------------------------------------
+static int elem_bytes(const edhmc_ctx* ctx) { return ctx->cdt == EDHMC_BF16 ? 2 : 4; }
+
+static LikParams lik_params(const edhmc_ctx* ctx) {
+  LikParams p;
+  p.lik = ctx->lik;
+  p.inv_s2 = 1.0f / (ctx->lik_scale * ctx->lik_scale);
+  p.ll_const = -log((double)ctx->lik_scale) - 0.5 * kLog2Pi;
+  return p;
+}
+
+static ChainArgs chain_args(edhmc_ctx* ctx) {
+  ChainArgs a = {};
+  a.C = ctx->C; a.D = ctx->D; a.D_pad = ctx->D_pad; a.C_pad = ctx->C_pad;
+  a.bf16 = ctx->cdt == EDHMC_BF16;
+  a.z = (float*)ctx->z.p; a.r = (float*)ctx->r.p; a.g = (float*)ctx->g.p;
+  a.z0 = (float*)ctx->z0.p; a.g0 = (float*)ctx->g0.p;
+  a.z_lp = ctx->z_lp.p;
+  a.logp = (double*)ctx->logp.p; a.K0 = (double*)ctx->K0.p;
+  a.ploc = (const float*)ctx->ploc.p; a.pscale = (const float*)ctx->pscale.p;
+  a.Gpart = (const float*)ctx->Gpart.p; a.ksplit = ctx->ksplit;
+  a.llpart = (const double*)ctx->llpart.p; a.ll_ld = ctx->n_tiles1; a.n_ll = ctx->n_tiles1;
+  a.ndiv = (int32_t*)ctx->ndiv.p;
+  return a;
+}
+
+template <typename T, typename Cfg, int PLANES, bool WR, bool LL>
+static int launch_gemm1(edhmc_ctx* ctx, hipStream_t st) {
+  const int eb = sizeof(T);
+  GemmShape g = {};
+  g.A = (const char*)ctx->z_lp.p;
+  g.lda_b = (int64_t)ctx->D_pad * eb;
+  g.a_plane_b = (int64_t)ctx->C_pad * ctx->D_pad * eb;
+  g.B = (const char*)ctx->Xr.p;
+  g.ldb_b = (int64_t)ctx->D_pad * eb;
+  g.m_tiles = ctx->C_pad / Cfg::BM;
+  g.n_tiles = (int)(ctx->N_pad / Cfg::BN);
+  g.k_stages = ctx->D_pad * eb / kStageBytes;
+  g.ksplit = 1;
+  g.stages_per_split = g.k_stages;
+  g.swz = 1;
+  EpiGLM<T, WR, LL> e;
+  e.y = (const float*)ctx->y.p;
+  e.N = ctx->N;
+  e.lp = lik_params(ctx);
+  e.Rt = (T*)ctx->Rt.p;
+  e.ldr = ctx->N_pad;
+  e.llpart = (double*)ctx->llpart.p;
+  e.ll_ld = ctx->n_tiles1;
+  hipLaunchKernelGGL((gemm_nt_kernel<T, Cfg, PLANES, EpiGLM<T, WR, LL>>), dim3(grid_size(g)),
+                     dim3(Cfg::kThreads), 0, st, g, e);
+  return check_launch(ctx, "gemm1");
+}
+
+template <typename T, typename Cfg>
+static int launch_gemm2_cfg(edhmc_ctx* ctx, hipStream_t st) {
+  const int eb = sizeof(T);
+  GemmShape g = {};
+  g.A = (const char*)ctx->Rt.p;
+  g.lda_b = ctx->N_pad * eb;
+  g.a_plane_b = 0;
+  g.B = (const char*)ctx->Xc.p;
+  g.ldb_b = ctx->N_pad * eb;
+  g.m_tiles = ctx->C_pad / Cfg::BM;
+  g.n_tiles = ctx->D_pad / Cfg::BN;
+  g.k_stages = (int)(ctx->N_pad * eb / kStageBytes);
+  g.ksplit = ctx->ksplit;
+  g.stages_per_split = (g.k_stages + g.ksplit - 1) / g.ksplit;
+  g.swz = (g.ksplit % 8 == 0) ? 2 : 0;
+  EpiStoreF32 e;
+  e.out = (float*)ctx->Gpart.p;
+  e.ldc = ctx->D_pad;
+  e.split_b = (int64_t)ctx->C_pad * ctx->D_pad;
+  hipLaunchKernelGGL((gemm_nt_kernel<T, Cfg, 1, EpiStoreF32>), dim3(grid_size(g)),
+                     dim3(Cfg::kThreads), 0, st, g, e);
+  return check_launch(ctx, "gemm2");
+}
+
+template <typename T>
+static int launch_gemm2(edhmc_ctx* ctx, hipStream_t st) {
+  if (ctx->D_pad % Cfg1::BN == 0) return launch_gemm2_cfg<T, Cfg1>(ctx, st);
+  return launch_gemm2_cfg<T, Cfg2S>(ctx, st);
+}
+
+// One gradient evaluation at z_lp.  split: use both bf16 planes; ll: write the
+// log-likelihood partials.
+static int grad_eval(edhmc_ctx* ctx, bool split, bool ll, hipStream_t st) {
+  if (ctx->cdt == EDHMC_BF16) {
+    if (split) {
+      if (ll) EDHMC_TRY((launch_gemm1<bf16_t, Cfg1, 2, true, true>(ctx, st)));
+      else EDHMC_TRY((launch_gemm1<bf16_t, Cfg1, 2, true, false>(ctx, st)));
+    } else {
+      if (ll) EDHMC_TRY((launch_gemm1<bf16_t, Cfg1, 1, true, true>(ctx, st)));
+      else EDHMC_TRY((launch_gemm1<bf16_t, Cfg1, 1, true, false>(ctx, st)));
+    }
+    return launch_gemm2<bf16_t>(ctx, st);
+  }
+  if (ll) EDHMC_TRY((launch_gemm1<float, Cfg1, 1, true, true>(ctx, st)));
+  else EDHMC_TRY((launch_gemm1<float, Cfg1, 1, true, false>(ctx, st)));
+  return launch_gemm2<float>(ctx, st);
+}
+
+// Forward only (log-likelihood partials, no residuals) at z_lp, exact form.
+int glm_forward_ll(edhmc_ctx* ctx, hipStream_t st) {
+  if (ctx->cdt == EDHMC_BF16) return launch_gemm1<bf16_t, Cfg1, 2, false, true>(ctx, st);
+  return launch_gemm1<float, Cfg1, 1, false, true>(ctx, st);
+}
+
+int glm_write_lp(edhmc_ctx* ctx, hipStream_t st) {
+  ChainArgs a = chain_args(ctx);
+  hipLaunchKernelGGL(chain_lp_kernel, dim3(ctx->C), dim3(kChainThreads), 0, st, a);
+  return check_launch(ctx, "chain_lp");
+}
+
+static int ensure_chains(edhmc_ctx* ctx, int C) {
+  if (C <= 0) return set_err(ctx, EDHMC_EINVAL, "number of chains must be positive, got %d", C);
+  if (ctx->C == C && ctx->z.p) return EDHMC_OK;
+  const int C_pad = (int)round_up(C, Cfg1::BM);
+  const size_t cd = (size_t)C * ctx->D_pad;
+  const int eb = elem_bytes(ctx);
+  ctx->C = C;
+  ctx->C_pad = C_pad;
+  ctx->state_valid = false;
+  ctx->mom_on = false;
+  ctx->mom_mean.release();
+  ctx->mom_m2.release();
+  EDHMC_TRY(dev_alloc(ctx, ctx->z, cd * 4));
+  EDHMC_TRY(dev_alloc(ctx, ctx->r, cd * 4));
+  EDHMC_TRY(dev_alloc(ctx, ctx->g, cd * 4));
+  EDHMC_TRY(dev_alloc(ctx, ctx->z0, cd * 4));
+  EDHMC_TRY(dev_alloc(ctx, ctx->g0, cd * 4));
+  const int planes = ctx->cdt == EDHMC_BF16 ? 2 : 1;
+  EDHMC_TRY(dev_alloc(ctx, ctx->z_lp, (size_t)planes * C_pad * ctx->D_pad * eb));
+  EDHMC_TRY(dev_alloc(ctx, ctx->logp, (size_t)C * 8));
+  EDHMC_TRY(dev_alloc(ctx, ctx->K0, (size_t)C * 8));
+  EDHMC_TRY(dev_alloc(ctx, ctx->ndiv, (size_t)C * 4));
+  EDHMC_TRY(dev_alloc(ctx, ctx->Rt, (size_t)C_pad * ctx->N_pad * eb, false));
+  // split-K for GEMM2: aim for >= 2048 workgroups, >= 8 stages per split.
+  const int bn2 = (ctx->D_pad % Cfg1::BN == 0) ? Cfg1::BN : Cfg2S::BN;
+  const int tiles2 = (C_pad / 128) * (ctx->D_pad / bn2);
+  const int kst = (int)(ctx->N_pad * eb / kStageBytes);
+  int ks = (2048 + tiles2 - 1) / tiles2;
+  ks = std::max(1, std::min(ks, std::min(64, kst / 8)));
+  if (ks >= 8) ks = ks / 8 * 8;
+  ctx->ksplit = ks;
+  EDHMC_TRY(dev_alloc(ctx, ctx->Gpart, (size_t)ks * C_pad * ctx->D_pad * 4, false));
+  ctx->n_tiles1 = (int)(ctx->N_pad / Cfg1::BN);
+  EDHMC_TRY(dev_alloc(ctx, ctx->llpart, (size_t)C_pad * ctx->n_tiles1 * 8, false));
+  return EDHMC_OK;
+}
+
+static int do_set_state(edhmc_ctx* ctx, const float* W, int C, hipStream_t st) {
+  if (!ctx->bound) return set_err(ctx, EDHMC_ESTATE, "no model bound");
+  if (!W) return set_err(ctx, EDHMC_EINVAL, "W is NULL");
+  EDHMC_TRY(ensure_chains(ctx, C));
+  EDHMC_TRY(dev_alloc(ctx, ctx->tmp_count, 4));
+  ChainArgs a = chain_args(ctx);
+  a.W_in = W;
+  hipLaunchKernelGGL(chain_load_kernel, dim3(C), dim3(kChainThreads), 0, st, a,
+                     (int*)ctx->tmp_count.p);
+  EDHMC_TRY(check_launch(ctx, "chain_load"));
+  int bad = 0;
+  if (hipMemcpyAsync(&bad, ctx->tmp_count.p, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return set_err(ctx, EDHMC_EHIP, "set_state: copy failed");
+  if (bad) {
+    ctx->state_valid = false;
+    return set_err(ctx, EDHMC_ENONFINITE, "chain positions contain %d non-finite rows", bad);
+  }
+  EDHMC_TRY(grad_eval(ctx, true, true, st));
+  a = chain_args(ctx);
+  hipLaunchKernelGGL(chain_eval_kernel, dim3(C), dim3(kChainThreads), 0, st, a);
+  EDHMC_TRY(check_launch(ctx, "chain_eval"));
+  ctx->state_valid = true;
+  return EDHMC_OK;
+}
+
+static int do_transition(edhmc_ctx* ctx, float eps, int L, const float* r0, const double* u,
+                         uint64_t seed, uint64_t counter, int32_t* accepted_out,
+                         float* sample_out, int32_t* n_accept, hipStream_t st) {
+  if (!ctx->bound || !ctx->state_valid)
+    return set_err(ctx, EDHMC_ESTATE, "no chain state: call edhmc_set_state first");
+  if (L < 1) return set_err(ctx, EDHMC_EINVAL, "n_steps must be >= 1, got %d", L);
+  if (!(std::isfinite(eps))) return set_err(ctx, EDHMC_EINVAL, "step_size must be finite");
+  ChainArgs a = chain_args(ctx);
+  a.half = (float)(0.5 * (double)eps);
+  a.full = eps;
+  a.r_fed = r0;
+  a.u_fed = u;
+  a.seed = seed;
+  a.counter = counter;
+  hipLaunchKernelGGL(chain_begin_kernel, dim3(ctx->C), dim3(kChainThreads), 0, st, a);
+  EDHMC_TRY(check_launch(ctx, "chain_begin"));
+  for (int k = 1; k <= L; ++k) {
+    const bool last = k == L;
+    EDHMC_TRY(grad_eval(ctx, last, last, st));
+    if (!last) {
+      hipLaunchKernelGGL(chain_mid_kernel, dim3(ctx->C), dim3(kChainThreads), 0, st, a);
+      EDHMC_TRY(check_launch(ctx, "chain_mid"));
+    }
+  }
+  a.sample_out = sample_out;
+  a.accepted_out = accepted_out;
+  a.n_accept = n_accept;
+  if (ctx->mom_on) {
+    ctx->mom_count += 1;
+    a.mom_mean = (double*)ctx->mom_mean.p;
+    a.mom_m2 = (double*)ctx->mom_m2.p;
+    a.mom_n = ctx->mom_count;
+  }
+  hipLaunchKernelGGL(chain_last_kernel, dim3(ctx->C), dim3(kChainThreads), 0, st, a);
+  return check_launch(ctx, "chain_last");
+}
+
+}  // namespace edhmc
+
+using namespace edhmc;
+
+edhmc_ctx::~edhmc_ctx() {
+  if (sur) surrogate_destroy(sur);
+  if (hier) hier_destroy(hier);
+  DevBuf* bufs[] = {&Xr, &Xc, &y, &ploc, &pscale, &z, &r, &g, &z0, &g0, &z_lp, &logp, &K0,
+                    &flag, &ndiv, &Rt, &Gpart, &llpart, &mom_mean, &mom_m2, &tmp_count};
+  for (DevBuf* b : bufs) b->release();
+}
+
+extern "C" {
+
+const char* edhmc_version(void) { return "edhmc 0.1 (gfx950)"; }
+
+int edhmc_create(int device, edhmc_ctx** out) {
+  if (!out) return EDHMC_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return EDHMC_EHIP;
+  if (device < 0 || device >= n) return EDHMC_EINVAL;
+  edhmc_ctx* c = new (std::nothrow) edhmc_ctx();
+  if (!c) return EDHMC_ENOMEM;
+  c->device = device;
+  *out = c;
+  return EDHMC_OK;
+}
+
+int edhmc_destroy(edhmc_ctx* ctx) {
+  if (!ctx) return EDHMC_OK;
+  DeviceGuard dg(ctx->device);
+  (void)hipDeviceSynchronize();
+  delete ctx;
+  return EDHMC_OK;
+}
+
+const char* edhmc_last_error(const edhmc_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int edhmc_bind_glm(edhmc_ctx* ctx, const void* X, int32_t x_in_dtype, const float* y, int64_t N,
+                   int32_t D, int32_t likelihood, float lik_scale, const float* prior_loc,
+                   const float* prior_scale, int32_t compute_dtype, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  if (!X || !y || !prior_loc || !prior_scale) return set_err(ctx, EDHMC_EINVAL, "NULL argument");
+  if (N <= 0 || D <= 0) return set_err(ctx, EDHMC_EINVAL, "need N > 0 and D > 0 (N=%lld D=%d)",
+                                       (long long)N, D);
+  if (x_in_dtype != EDHMC_F32 && x_in_dtype != EDHMC_BF16)
+    return set_err(ctx, EDHMC_EINVAL, "bad x_in_dtype %d", x_in_dtype);
+  if (compute_dtype != EDHMC_F32 && compute_dtype != EDHMC_BF16)
+    return set_err(ctx, EDHMC_EINVAL, "bad compute_dtype %d", compute_dtype);
+  if (likelihood != EDHMC_LIK_BERNOULLI && likelihood != EDHMC_LIK_NORMAL)
+    return set_err(ctx, EDHMC_EUNSUPPORTED, "unsupported likelihood %d", likelihood);
+  if (likelihood == EDHMC_LIK_NORMAL && !(lik_scale > 0.0f && std::isfinite(lik_scale)))
+    return set_err(ctx, EDHMC_EINVAL, "likelihood scale must be positive and finite");
+  if (D > 65536) return set_err(ctx, EDHMC_EUNSUPPORTED, "D=%d too large", D);
+  ctx->bound = false;
+  ctx->state_valid = false;
+  ctx->C = 0;
+  ctx->z.release();
+  ctx->cdt = compute_dtype;
+  ctx->lik = likelihood;
+  ctx->lik_scale = lik_scale;
+  ctx->N = N;
+  ctx->N_pad = round_up(N, Cfg1::BN);
+  ctx->D = D;
+  ctx->D_pad = (int)round_up(D, 32);
+  const int eb = elem_bytes(ctx);
+  EDHMC_TRY(dev_alloc(ctx, ctx->Xr, (size_t)ctx->N_pad * ctx->D_pad * eb, false));
+  EDHMC_TRY(dev_alloc(ctx, ctx->Xc, (size_t)ctx->N_pad * ctx->D_pad * eb, false));
+  EDHMC_TRY(dev_alloc(ctx, ctx->y, (size_t)ctx->N_pad * 4, false));
+  EDHMC_TRY(dev_alloc(ctx, ctx->ploc, (size_t)ctx->D_pad * 4, false));
+  EDHMC_TRY(dev_alloc(ctx, ctx->pscale, (size_t)ctx->D_pad * 4, false));
+  EDHMC_TRY(dev_alloc(ctx, ctx->tmp_count, 4));
+  int* bad = (int*)ctx->tmp_count.p;
+  dim3 grid(ctx->D_pad / 32, (unsigned)(ctx->N_pad / 32));
+  if (ctx->N_pad / 32 > 2147483647LL) return set_err(ctx, EDHMC_EUNSUPPORTED, "N too large");
+  if (x_in_dtype == EDHMC_F32) {
+    if (compute_dtype == EDHMC_F32)
+      hipLaunchKernelGGL((layout_x_kernel<float, float>), grid, dim3(256), 0, st, (const float*)X,
+                         N, D, (float*)ctx->Xr.p, (float*)ctx->Xc.p, ctx->N_pad, ctx->D_pad, bad);
+    else
+      hipLaunchKernelGGL((layout_x_kernel<float, bf16_t>), grid, dim3(256), 0, st,
+                         (const float*)X, N, D, (bf16_t*)ctx->Xr.p, (bf16_t*)ctx->Xc.p,
+                         ctx->N_pad, ctx->D_pad, bad);
+  } else {
+    if (compute_dtype == EDHMC_F32)
+      hipLaunchKernelGGL((layout_x_kernel<bf16_t, float>), grid, dim3(256), 0, st,
+                         (const bf16_t*)X, N, D, (float*)ctx->Xr.p, (float*)ctx->Xc.p,
+                         ctx->N_pad, ctx->D_pad, bad);
+    else
+      hipLaunchKernelGGL((layout_x_kernel<bf16_t, bf16_t>), grid, dim3(256), 0, st,
+                         (const bf16_t*)X, N, D, (bf16_t*)ctx->Xr.p, (bf16_t*)ctx->Xc.p,
+                         ctx->N_pad, ctx->D_pad, bad);
+  }
+  EDHMC_TRY(check_launch(ctx, "layout_x"));
+  auto padc = [&](const float* src, int64_t n, DevBuf& dst, int64_t npad, float fill) {
+    hipLaunchKernelGGL(pad_copy_kernel, dim3((unsigned)((npad + 255) / 256)), dim3(256), 0, st,
+                       src, n, (float*)dst.p, npad, fill, bad);
+    return check_launch(ctx, "pad_copy");
+  };
+  EDHMC_TRY(padc(y, N, ctx->y, ctx->N_pad, 0.0f));
+  EDHMC_TRY(padc(prior_loc, D, ctx->ploc, ctx->D_pad, 0.0f));
+  EDHMC_TRY(padc(prior_scale, D, ctx->pscale, ctx->D_pad, 1.0f));
+  int nbad = 0;
+  if (hipMemcpyAsync(&nbad, bad, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return set_err(ctx, EDHMC_EHIP, "bind: copy failed");
+  if (nbad) return set_err(ctx, EDHMC_ENONFINITE, "data contains non-finite values (%d blocks)", nbad);
+  ctx->bound = true;
+  return EDHMC_OK;
+}
+
+int edhmc_set_state(edhmc_ctx* ctx, const float* W, int32_t C, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  return do_set_state(ctx, W, C, (hipStream_t)stream);
+}
+
+int edhmc_get_state(edhmc_ctx* ctx, float* W_out, double* logp_out, float* grad_out, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  if (!ctx->state_valid) return set_err(ctx, EDHMC_ESTATE, "no chain state");
+  const int64_t n = (int64_t)ctx->C * ctx->D;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  if (W_out) {
+    hipLaunchKernelGGL(copy_out_kernel, dim3(blocks), dim3(256), 0, st, (const float*)ctx->z.p,
+                       ctx->C, ctx->D, ctx->D_pad, W_out);
+    EDHMC_TRY(check_launch(ctx, "copy_out"));
+  }
+  if (grad_out) {
+    hipLaunchKernelGGL(copy_out_kernel, dim3(blocks), dim3(256), 0, st, (const float*)ctx->g.p,
+                       ctx->C, ctx->D, ctx->D_pad, grad_out);
+    EDHMC_TRY(check_launch(ctx, "copy_out"));
+  }
+  if (logp_out && hipMemcpyAsync(logp_out, ctx->logp.p, (size_t)ctx->C * 8,
+                                 hipMemcpyDeviceToDevice, st) != hipSuccess)
+    return set_err(ctx, EDHMC_EHIP, "get_state copy failed");
+  return EDHMC_OK;
+}
+
+int edhmc_logp_grad(edhmc_ctx* ctx, const float* W, int32_t C, float* grad_out, double* logp_out,
+                    void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  EDHMC_TRY(do_set_state(ctx, W, C, (hipStream_t)stream));
+  return edhmc_get_state(ctx, nullptr, logp_out, grad_out, stream);
+}
+
+int edhmc_transition(edhmc_ctx* ctx, float eps, int32_t n_steps, const float* r0, const double* u,
+                     uint64_t seed, uint64_t counter, int32_t* accepted_out, float* sample_out,
+                     int32_t* n_accept, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  return do_transition(ctx, eps, n_steps, r0, u, seed, counter, accepted_out, sample_out,
+                       n_accept, (hipStream_t)stream);
+}
+
+int edhmc_run(edhmc_ctx* ctx, float eps, int32_t n_steps, float* params, int64_t T, int64_t t0,
+              int64_t n_iter, uint64_t seed, uint64_t counter0, int32_t* n_accept, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  if (!params) return set_err(ctx, EDHMC_EINVAL, "params is NULL");
+  if (t0 < 0 || n_iter < 0 || t0 + n_iter > T)
+    return set_err(ctx, EDHMC_EINVAL, "iterations [%lld, %lld) outside params of %lld rows",
+                   (long long)t0, (long long)(t0 + n_iter), (long long)T);
+  for (int64_t t = t0; t < t0 + n_iter; ++t) {
+    EDHMC_TRY(do_transition(ctx, eps, n_steps, nullptr, nullptr, seed, counter0 + (uint64_t)t,
+                            nullptr, params + t * (int64_t)ctx->C * ctx->D, n_accept,
+                            (hipStream_t)stream));
+  }
+  return EDHMC_OK;
+}
+
+int edhmc_moments_enable(edhmc_ctx* ctx, int32_t enable) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  if (!enable) {
+    ctx->mom_on = false;
+    ctx->mom_mean.release();
+    ctx->mom_m2.release();
+    return EDHMC_OK;
+  }
+  if (!ctx->z.p) return set_err(ctx, EDHMC_ESTATE, "no chains");
+  const size_t bytes = (size_t)ctx->C * ctx->D_pad * 8;
+  EDHMC_TRY(dev_alloc(ctx, ctx->mom_mean, bytes));
+  EDHMC_TRY(dev_alloc(ctx, ctx->mom_m2, bytes));
+  ctx->mom_count = 0;
+  ctx->mom_on = true;
+  return EDHMC_OK;
+}
+
+int edhmc_moments_get(edhmc_ctx* ctx, double* mean_out, double* m2_out, int64_t* count_out,
+                      void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  if (!ctx->mom_on) return set_err(ctx, EDHMC_ESTATE, "moments not enabled");
+  const int64_t n = (int64_t)ctx->C * ctx->D;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  if (mean_out) {
+    hipLaunchKernelGGL(copy_out_f64_kernel, dim3(blocks), dim3(256), 0, st,
+                       (const double*)ctx->mom_mean.p, ctx->C, ctx->D, ctx->D_pad, mean_out);
+    EDHMC_TRY(check_launch(ctx, "copy_out_f64"));
+  }
+  if (m2_out) {
+    hipLaunchKernelGGL(copy_out_f64_kernel, dim3(blocks), dim3(256), 0, st,
+                       (const double*)ctx->mom_m2.p, ctx->C, ctx->D, ctx->D_pad, m2_out);
+    EDHMC_TRY(check_launch(ctx, "copy_out_f64"));
+  }
+  if (count_out) *count_out = ctx->mom_count;
+  return EDHMC_OK;
+}
+
+int edhmc_divergences(edhmc_ctx* ctx, int32_t* out, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  if (!ctx->ndiv.p || !out) return set_err(ctx, EDHMC_ESTATE, "no chains");
+  if (hipMemcpyAsync(out, ctx->ndiv.p, (size_t)ctx->C * 4, hipMemcpyDeviceToDevice,
+                     (hipStream_t)stream) != hipSuccess)
+    return set_err(ctx, EDHMC_EHIP, "copy failed");
+  return EDHMC_OK;
+}
+
+int edhmc_draw_momenta(edhmc_ctx* ctx, int32_t C, int32_t D, uint64_t seed, uint64_t counter,
+                       float* r_out, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  if (C <= 0 || D <= 0 || !r_out) return set_err(ctx, EDHMC_EINVAL, "bad arguments");
+  hipLaunchKernelGGL(rng_momenta_kernel, dim3(C), dim3(128), 0, (hipStream_t)stream, C, D, seed,
+                     counter, r_out);
+  return check_launch(ctx, "rng_momenta");
+}
+
+int edhmc_draw_uniforms(edhmc_ctx* ctx, int32_t C, uint64_t seed, uint64_t counter, double* u_out,
+                        void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  if (C <= 0 || !u_out) return set_err(ctx, EDHMC_EINVAL, "bad arguments");
+  hipLaunchKernelGGL(rng_uniforms_kernel, dim3((C + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                     C, seed, counter, u_out);
+  return check_launch(ctx, "rng_uniforms");
+}
+
+int edhmc_autocov(edhmc_ctx* ctx, const float* s, int64_t T, int32_t C, int32_t D, int32_t max_lag,
+                  double* acov_out, double* chain_mean_out, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  if (!s || !acov_out || T < 2 || C <= 0 || D <= 0 || max_lag < 0)
+    return set_err(ctx, EDHMC_EINVAL, "bad arguments");
+  if (max_lag >= T) max_lag = (int)(T - 1);
+  if (hipMemsetAsync(acov_out, 0, (size_t)(max_lag + 1) * D * 8, st) != hipSuccess)
+    return set_err(ctx, EDHMC_EHIP, "memset failed");
+  const int64_t n = (int64_t)C * D;
+  hipLaunchKernelGGL(autocov_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s, T, C,
+                     D, max_lag, acov_out, chain_mean_out);
+  return check_launch(ctx, "autocov");
+}
+
+}  // extern "C"

@@ -1,0 +1,244 @@
+// MFMA "NT" GEMM for gfx950:  C[m, n] = sum_k A[m, k] * B[n, k]
+//
+// Both operands are row-major with K contiguous, so every lane feeds the
+// matrix core from one 16-byte LDS read per operand and K-chunk:
+//   * bf16: v_mfma_f32_32x32x16_bf16, the 16 bytes are 8 consecutive k;
+//   * fp32: v_mfma_f32_32x32x2_f32 issued 4 times, element s of the 16 bytes
+//     feeds issue s.  Lanes 0-31 and 32-63 read the two 16-byte halves of a
+//     32-byte K-chunk, so A and B see the same k <-> (lane, element)
+//     assignment and the contraction covers each k exactly once.
+//
+// Work decomposition: a workgroup of WAVES_M x WAVES_N wave64s owns a
+// BM x BN output tile (each wave WM x WN blocks of 32 x 32) and a range of
+// K stages (split-K).  Stages of 64 bytes per row are double-buffered in LDS
+// (row stride 80 bytes: the 8/16 lanes served per LDS cycle by a
+// ds_read_b128 hit disjoint banks).  The next stage's global loads are in
+// flight while the current stage feeds the MFMAs.
+//
+// Tile order is XCD-aware: consecutive workgroup ids go round-robin to the
+// 8 XCDs, so ids are mapped such that all workgroups that read the same B
+// tile (mode 1) or the same K split (mode 2) land on one XCD and share its L2.
+#pragma once
+
+#include "common.h"
+
+namespace edhmc {
+
+constexpr int kStageBytes = 64;
+constexpr int kRowStride = 80;
+
+struct GemmShape {
+  const char* A;      // [planes][M][K]
+  int64_t lda_b;      // bytes between rows of A
+  int64_t a_plane_b;  // bytes between planes of A
+  const char* B;      // [N][K]
+  int64_t ldb_b;
+  int m_tiles, n_tiles;
+  int k_stages;          // K bytes / 64
+  int ksplit;            // number of K ranges
+  int stages_per_split;
+  int swz;               // 0: plain, 1: group by n tile, 2: group by K split
+};
+
+__device__ inline bool map_tile(const GemmShape& g, int id, int& tm, int& tn, int& ks) {
+  if (g.swz == 1) {
+    int xcd = id & 7, local = id >> 3;
+    int grp = local / g.m_tiles;
+    tm = local - grp * g.m_tiles;
+    tn = xcd + 8 * grp;
+    ks = 0;
+    return tn < g.n_tiles;
+  }
+  int tiles = g.m_tiles * g.n_tiles;
+  int t;
+  if (g.swz == 2) {
+    int xcd = id & 7, local = id >> 3;
+    int grp = local / tiles;
+    t = local - grp * tiles;
+    ks = xcd + 8 * grp;
+    if (ks >= g.ksplit) return false;
+  } else {
+    ks = id / tiles;
+    t = id - ks * tiles;
+  }
+  tm = t % g.m_tiles;
+  tn = t / g.m_tiles;
+  return true;
+}
+
+inline int grid_size(const GemmShape& g) {
+  if (g.swz == 1) return g.m_tiles * ((g.n_tiles + 7) / 8) * 8;
+  if (g.swz == 2) return g.m_tiles * g.n_tiles * ((g.ksplit + 7) / 8) * 8;
+  return g.m_tiles * g.n_tiles * g.ksplit;
+}
+
+template <typename T> struct Mfma;
+
+template <> struct Mfma<bf16_t> {
+  __device__ static inline void chunk(f32x16_t& acc, const uint4& a, const uint4& b) {
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                  __builtin_bit_cast(bf16x8_t, b),
+                                                  acc, 0, 0, 0);
+  }
+};
+
+template <> struct Mfma<float> {
+  __device__ static inline void chunk(f32x16_t& acc, const uint4& a, const uint4& b) {
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(__builtin_bit_cast(float, a.x),
+                                               __builtin_bit_cast(float, b.x), acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(__builtin_bit_cast(float, a.y),
+                                               __builtin_bit_cast(float, b.y), acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(__builtin_bit_cast(float, a.z),
+                                               __builtin_bit_cast(float, b.z), acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(__builtin_bit_cast(float, a.w),
+                                               __builtin_bit_cast(float, b.w), acc, 0, 0, 0);
+  }
+};
+
+// Row of 32x32 accumulator element r held by `lane` (column is lane & 31).
+__device__ inline int acc_row(int r, int lane) {
+  return ((r >> 2) << 3) + ((lane >> 5) << 2) + (r & 3);
+}
+
+template <int WM, int WN, int WAVES_M, int WAVES_N>
+struct TileCfg {
+  static constexpr int kWM = WM, kWN = WN, kWavesM = WAVES_M, kWavesN = WAVES_N;
+  static constexpr int kThreads = 64 * WAVES_M * WAVES_N;
+  static constexpr int BM = 32 * WM * WAVES_M;
+  static constexpr int BN = 32 * WN * WAVES_N;
+};
+
+// Epilogue context handed to the epilogue functor.
+struct EpiPos {
+  int m0, n0;     // global row / column of this wave's first accumulator block
+  int lane, wm, wn, tm, tn, ks;
+  char* smem;     // free LDS (>= the main loop's footprint) after a barrier
+};
+
+template <typename T, typename Cfg, int A_PLANES, typename Epi>
+__global__ void __launch_bounds__(Cfg::kThreads)
+gemm_nt_kernel(GemmShape g, Epi epi) {
+  constexpr int NT = Cfg::kThreads;
+  constexpr int WM = Cfg::kWM, WN = Cfg::kWN, WAVES_M = Cfg::kWavesM;
+  constexpr int BM = Cfg::BM, BN = Cfg::BN;
+  constexpr int AR = BM * A_PLANES;
+  constexpr int ROWS = AR + BN;
+  constexpr int CPR = kStageBytes / 16;          // 16-byte chunks per row
+  constexpr int CHUNKS = ROWS * CPR;
+  constexpr int PER_T = (CHUNKS + NT - 1) / NT;
+  constexpr int BUF = ROWS * kRowStride;
+  __shared__ __align__(16) char smem[2 * BUF];
+
+  int tm, tn, ks;
+  if (!map_tile(g, blockIdx.x, tm, tn, ks)) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WAVES_M, wn = wave / WAVES_M;
+  const int s_begin = ks * g.stages_per_split;
+  const int s_end = min(g.k_stages, s_begin + g.stages_per_split);
+
+  const char* src[PER_T];
+  int dst[PER_T];
+#pragma unroll
+  for (int q = 0; q < PER_T; ++q) {
+    int c = tid + q * NT;
+    if (c >= CHUNKS) c = CHUNKS - 1;  // duplicate the last chunk (same data)
+    int row = c / CPR, col = c - row * CPR;
+    if (row < AR) {
+      int p = row / BM, r = row - p * BM;
+      src[q] = g.A + p * g.a_plane_b + (int64_t)(tm * BM + r) * g.lda_b + col * 16;
+    } else {
+      int r = row - AR;
+      src[q] = g.B + (int64_t)(tn * BN + r) * g.ldb_b + col * 16;
+    }
+    dst[q] = row * kRowStride + col * 16;
+  }
+
+  f32x16_t acc[WM][WN];
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+  uint4 pre[PER_T];
+  if (s_begin < s_end) {
+#pragma unroll
+    for (int q = 0; q < PER_T; ++q)
+      pre[q] = *(const uint4*)(src[q] + (int64_t)s_begin * kStageBytes);
+#pragma unroll
+    for (int q = 0; q < PER_T; ++q) *(uint4*)(smem + dst[q]) = pre[q];
+  }
+  __syncthreads();
+
+  const int lrow = lane & 31, lhalf = (lane >> 5) * 16;
+  for (int s = s_begin; s < s_end; ++s) {
+    const int buf = (s - s_begin) & 1;
+    const bool more = s + 1 < s_end;
+    if (more) {
+#pragma unroll
+      for (int q = 0; q < PER_T; ++q)
+        pre[q] = *(const uint4*)(src[q] + (int64_t)(s + 1) * kStageBytes);
+    }
+    const char* base = smem + buf * BUF;
+#pragma unroll
+    for (int kc = 0; kc < kStageBytes / 32; ++kc) {
+      uint4 a[A_PLANES][WM], b[WN];
+#pragma unroll
+      for (int p = 0; p < A_PLANES; ++p)
+#pragma unroll
+        for (int i = 0; i < WM; ++i)
+          a[p][i] = *(const uint4*)(base + (p * BM + wm * 32 * WM + 32 * i + lrow) * kRowStride +
+                                    kc * 32 + lhalf);
+#pragma unroll
+      for (int j = 0; j < WN; ++j)
+        b[j] = *(const uint4*)(base + (AR + wn * 32 * WN + 32 * j + lrow) * kRowStride +
+                               kc * 32 + lhalf);
+#pragma unroll
+      for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j)
+#pragma unroll
+          for (int p = 0; p < A_PLANES; ++p) Mfma<T>::chunk(acc[i][j], a[p][i], b[j]);
+    }
+    if (more) {
+      char* nb = smem + (buf ^ 1) * BUF;
+#pragma unroll
+      for (int q = 0; q < PER_T; ++q) *(uint4*)(nb + dst[q]) = pre[q];
+    }
+    __syncthreads();
+  }
+
+  EpiPos pos;
+  pos.m0 = tm * BM + wm * 32 * WM;
+  pos.n0 = tn * BN + wn * 32 * WN;
+  pos.lane = lane; pos.wm = wm; pos.wn = wn; pos.tm = tm; pos.tn = tn; pos.ks = ks;
+  pos.smem = smem;
+  epi.template apply<Cfg>(acc, pos);
+}
+
+// ---------------------------------------------------------------------------
+// Epilogue: store the fp32 tile (split-K partials: out[ks][m][n]).
+// ---------------------------------------------------------------------------
+struct EpiStoreF32 {
+  float* out;
+  int64_t ldc;        // elements between rows
+  int64_t split_b;    // elements between split planes
+  template <typename Cfg>
+  __device__ void apply(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p) const {
+    float* o = out + (int64_t)p.ks * split_b;
+#pragma unroll
+    for (int i = 0; i < Cfg::kWM; ++i)
+#pragma unroll
+      for (int j = 0; j < Cfg::kWN; ++j) {
+        const int n = p.n0 + 32 * j + (p.lane & 31);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int m = p.m0 + 32 * i + acc_row(r, p.lane);
+          o[(int64_t)m * ldc + n] = acc[i][j][r];
+        }
+      }
+  }
+};
+
+}  // namespace edhmc

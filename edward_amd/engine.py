@@ -1,0 +1,167 @@
+"""Device engines: thin owners of an edhmc context.
+
+torch is used only to hold device memory and to name the stream; all
+arithmetic of the sampler runs in libedhmc.so's kernels.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from edward_amd import _lib as L
+
+
+def _dev_tensor(x, dtype, device):
+  if isinstance(x, torch.Tensor):
+    return x.to(device=device, dtype=dtype).contiguous()
+  return torch.as_tensor(np.asarray(x), dtype=dtype, device=device).contiguous()
+
+
+def require_gpu():
+  if not torch.cuda.is_available():
+    raise RuntimeError('edward_amd needs a ROCm GPU (MI355X/gfx950); none is visible')
+
+
+class Context(object):
+  """An edhmc_ctx on one device."""
+
+  def __init__(self, device=0):
+    require_gpu()
+    self.lib = L.load()
+    self.device = torch.device('cuda', device)
+    h = ctypes.c_void_p()
+    L.check(self.lib.edhmc_create(device, ctypes.byref(h)))
+    self.h = h
+
+  def close(self):
+    if getattr(self, 'h', None):
+      torch.cuda.synchronize(self.device)
+      self.lib.edhmc_destroy(self.h)
+      self.h = None
+
+  def __del__(self):
+    try:
+      self.close()
+    except Exception:
+      pass
+
+  def check(self, rc):
+    L.check(rc, self.h)
+
+  @property
+  def stream(self):
+    return L.stream_ptr(torch.cuda.current_stream(self.device))
+
+
+class GLMEngine(Context):
+  """HMC over a GLM  y ~ Lik(X w),  w ~ Normal(loc, scale)  for C chains.
+
+  precision: 'fp32' (fp32 X and MFMA) or 'bf16' (bf16 X, bf16 MFMA with fp32
+  accumulation, hi/lo split positions for the log joint).
+  """
+
+  def __init__(self, X, y, prior_loc=0.0, prior_scale=1.0, likelihood='bernoulli',
+               lik_scale=1.0, precision='fp32', device=0):
+    super(GLMEngine, self).__init__(device)
+    dev = self.device
+    if isinstance(X, torch.Tensor) and X.dtype == torch.bfloat16:
+      Xd, xin = X.to(dev).contiguous(), L.BF16
+    else:
+      Xd, xin = _dev_tensor(X, torch.float32, dev), L.F32
+    if Xd.dim() != 2:
+      raise ValueError('X must be 2-D, got shape %s' % (tuple(Xd.shape),))
+    N, D = Xd.shape
+    yd = _dev_tensor(y, torch.float32, dev).reshape(-1)
+    if yd.numel() != N:
+      raise ValueError('y has %d entries for %d rows of X' % (yd.numel(), N))
+    loc = _dev_tensor(np.broadcast_to(np.asarray(
+        prior_loc.cpu() if isinstance(prior_loc, torch.Tensor) else prior_loc,
+        np.float32), (D,)), torch.float32, dev)
+    scale = _dev_tensor(np.broadcast_to(np.asarray(
+        prior_scale.cpu() if isinstance(prior_scale, torch.Tensor) else prior_scale,
+        np.float32), (D,)), torch.float32, dev)
+    lik = {'bernoulli': L.LIK_BERNOULLI, 'normal': L.LIK_NORMAL}[likelihood]
+    cdt = {'fp32': L.F32, 'bf16': L.BF16}[precision]
+    self.N, self.D = int(N), int(D)
+    self.precision = precision
+    self.likelihood = likelihood
+    self.C = None
+    self.check(self.lib.edhmc_bind_glm(self.h, L.ptr(Xd), xin, L.ptr(yd), N, D, lik,
+                                       float(lik_scale), L.ptr(loc), L.ptr(scale), cdt,
+                                       self.stream))
+    torch.cuda.synchronize(dev)
+
+  # -- state ---------------------------------------------------------------
+  def set_state(self, W):
+    W = _dev_tensor(W, torch.float32, self.device)
+    if W.dim() == 1:
+      W = W[None]
+    if W.shape[1] != self.D:
+      raise ValueError('positions have %d coordinates, model has %d' % (W.shape[1], self.D))
+    self.C = int(W.shape[0])
+    self.check(self.lib.edhmc_set_state(self.h, L.ptr(W), self.C, self.stream))
+
+  def get_state(self):
+    W = torch.empty(self.C, self.D, device=self.device)
+    lp = torch.empty(self.C, dtype=torch.float64, device=self.device)
+    g = torch.empty(self.C, self.D, device=self.device)
+    self.check(self.lib.edhmc_get_state(self.h, L.ptr(W), L.ptr(lp), L.ptr(g), self.stream))
+    return W, lp, g
+
+  def logp_grad(self, W):
+    W = _dev_tensor(W, torch.float32, self.device)
+    if W.dim() == 1:
+      W = W[None]
+    C = int(W.shape[0])
+    g = torch.empty(C, self.D, device=self.device)
+    lp = torch.empty(C, dtype=torch.float64, device=self.device)
+    self.C = C
+    self.check(self.lib.edhmc_logp_grad(self.h, L.ptr(W), C, L.ptr(g), L.ptr(lp), self.stream))
+    return lp, g
+
+  # -- sampling ------------------------------------------------------------
+  def transition(self, step_size, n_steps, momenta=None, uniforms=None, seed=0, counter=0,
+                 sample_out=None, n_accept=None, accepted_out=None):
+    r0 = None if momenta is None else _dev_tensor(momenta, torch.float32, self.device)
+    u = None if uniforms is None else _dev_tensor(uniforms, torch.float64, self.device)
+    self.check(self.lib.edhmc_transition(
+        self.h, float(step_size), int(n_steps), L.ptr(r0), L.ptr(u), int(seed), int(counter),
+        L.ptr(accepted_out), L.ptr(sample_out), L.ptr(n_accept), self.stream))
+
+  def run(self, params, step_size, n_steps, t0=0, n_iter=None, seed=0, counter0=0,
+          n_accept=None):
+    T = int(params.shape[0])
+    if n_iter is None:
+      n_iter = T - t0
+    self.check(self.lib.edhmc_run(self.h, float(step_size), int(n_steps), L.ptr(params), T,
+                                  int(t0), int(n_iter), int(seed), int(counter0),
+                                  L.ptr(n_accept), self.stream))
+
+  def moments_enable(self, on=True):
+    self.check(self.lib.edhmc_moments_enable(self.h, 1 if on else 0))
+
+  def moments(self):
+    mean = torch.empty(self.C, self.D, dtype=torch.float64, device=self.device)
+    m2 = torch.empty(self.C, self.D, dtype=torch.float64, device=self.device)
+    n = ctypes.c_int64()
+    self.check(self.lib.edhmc_moments_get(self.h, L.ptr(mean), L.ptr(m2), ctypes.byref(n),
+                                          self.stream))
+    return mean, m2, int(n.value)
+
+  def divergences(self):
+    out = torch.empty(self.C, dtype=torch.int32, device=self.device)
+    self.check(self.lib.edhmc_divergences(self.h, L.ptr(out), self.stream))
+    return out
+
+  def draw_momenta(self, C, seed, counter):
+    out = torch.empty(C, self.D, device=self.device)
+    self.check(self.lib.edhmc_draw_momenta(self.h, C, self.D, int(seed), int(counter),
+                                           L.ptr(out), self.stream))
+    return out
+
+  def draw_uniforms(self, C, seed, counter):
+    out = torch.empty(C, dtype=torch.float64, device=self.device)
+    self.check(self.lib.edhmc_draw_uniforms(self.h, C, int(seed), int(counter), L.ptr(out),
+                                            self.stream))
+    return out

@@ -1,0 +1,289 @@
+"""Inference -> MonteCarlo -> HMC with Edward 1.3.5's control flow.
+
+The class hierarchy, constructor checks, ``initialize``/``update``/``run``/
+``print_progress``/``finalize`` and the ``t``/``n_accept``/``reset`` state
+follow ``edward/inferences/inference.py:54-338``,
+``monte_carlo.py:61-170`` and ``hmc.py:42-130``.  ``HMC.build_update`` does
+not build a graph: it compiles the model to a device engine, and ``update``
+runs one fused transition of every chain through the C ABI.
+
+Deliberate differences (each has a test in tests/test_surface.py):
+  * ``initialize`` returns ``self``, so ``HMC(...).initialize(eps, L).run()``
+    chains; ``run()`` without keyword arguments after an explicit
+    ``initialize`` keeps those hyperparameters (the reference's ``run`` always
+    re-initialises, ``inference.py:125``, silently resetting eps/L to
+    0.25/2).
+  * ``accept_rate`` is ``n_accept / t`` read after the increment (the
+    reference fetches it in the same session call as the increment,
+    ``monte_carlo.py:139-140``, which races and divides by 0 at t = 0).
+  * ``Empirical(params[T, C, D])`` for a ``[D]`` latent runs C independent
+    chains (a shape error in the reference, ``util/random_variables.py:78-83``).
+  * A non-finite proposal is rejected and counted (``divergences``); with
+    ``debug=True`` ``update`` raises ``FloatingPointError`` (the reference's
+    ``tf.add_check_numerics_ops``, ``inference.py:279-281``).
+"""
+
+import json
+import os
+import time
+
+import numpy as np
+import torch
+
+from edward_amd import models as M
+from edward_amd import compile as _compile
+from edward_amd.progbar import Progbar
+
+
+def check_latent_vars(latent_vars):
+  """``util/random_variables.py:64-83``."""
+  if not isinstance(latent_vars, dict):
+    raise TypeError('latent_vars must have type dict.')
+  for key, value in latent_vars.items():
+    if not isinstance(key, (M.RandomVariable, M.Node)):
+      raise TypeError('Latent variable key has an invalid type: {}'.format(type(key)))
+    if not isinstance(value, (M.RandomVariable, M.Node)):
+      raise TypeError('Latent variable value has an invalid type: {}'.format(type(value)))
+    if isinstance(value, M.Empirical) and isinstance(key, M.RandomVariable):
+      ev, ks = tuple(value.event_shape), tuple(key.shape)
+      if ev != ks and not (len(ev) == len(ks) + 1 and ev[1:] == ks):
+        raise TypeError('Key-value pair in latent_vars does not have same shape: '
+                        '{}, {}'.format(ks, ev))
+      if value.dtype != key.dtype:
+        raise TypeError('Key-value pair in latent_vars does not have same dtype: '
+                        '{}, {}'.format(key.dtype, value.dtype))
+
+
+def check_data(data):
+  """``util/random_variables.py:21-61``."""
+  if not isinstance(data, dict):
+    raise TypeError('data must have type dict.')
+  for key, value in data.items():
+    if isinstance(key, M.Placeholder):
+      if isinstance(value, (M.RandomVariable, M.Node)):
+        raise TypeError('The value of a feed cannot be a ed.RandomVariable object. '
+                        'Acceptable feed values include Python scalars, strings, lists, '
+                        'numpy ndarrays, or TensorHandles.')
+    elif isinstance(key, M.RandomVariable):
+      if isinstance(value, (M.RandomVariable, M.Node)):
+        continue
+      if isinstance(value, (float, int, list, np.ndarray, np.number, torch.Tensor)):
+        shp = tuple(value.shape) if hasattr(value, 'shape') else np.shape(value)
+        if key.shape and tuple(shp) != tuple(key.shape):
+          raise TypeError('Key-value pair in data does not have same shape: '
+                          '{}, {}'.format(key.shape, shp))
+      else:
+        raise TypeError('Data value has an invalid type: {}'.format(type(value)))
+    else:
+      raise TypeError('Data key has an invalid type: {}'.format(type(key)))
+
+
+class Inference(object):
+  """``edward/inferences/inference.py:19-338``."""
+
+  def __init__(self, latent_vars=None, data=None):
+    if latent_vars is None:
+      latent_vars = {}
+    if data is None:
+      data = {}
+    check_latent_vars(latent_vars)
+    self.latent_vars = latent_vars
+    check_data(data)
+    self.data = dict(data)
+    self._initialized = False
+
+  def run(self, variables=None, use_coordinator=True, *args, **kwargs):
+    """``inference.py:97-154``: initialize, (re)initialise variables, loop
+    ``update``/``print_progress`` n_iter times, ``finalize``."""
+    if args or kwargs or not self._initialized:
+      self.initialize(*args, **kwargs)
+    self._initialize_variables(variables)
+    for _ in range(self.n_iter):
+      info_dict = self.update()
+      self.print_progress(info_dict)
+    self.finalize()
+
+  def initialize(self, n_iter=1000, n_print=None, scale=None, auto_transform=True,
+                 logdir=None, log_timestamp=True, log_vars=None, debug=False):
+    """``inference.py:157-285``."""
+    self.n_iter = int(n_iter)
+    self.n_print = int(n_iter / 100) if n_print is None else int(n_print)
+    self.progbar = Progbar(self.n_iter)
+    self.t = 0
+    if scale is None:
+      scale = {}
+    elif not isinstance(scale, dict):
+      raise TypeError('scale must be a dict object.')
+    self.scale = scale
+    self.transformations = {}
+    if auto_transform:
+      self.latent_vars_unconstrained = {}
+      for z, qz in self.latent_vars.items():
+        if getattr(z, 'support', None) not in (None, 'real', 'multivariate_real') and \
+                getattr(qz, 'support', None) != 'point':
+          raise NotImplementedError(
+              'auto_transform of support %r is not implemented; only real-support '
+              'latents (identity transform) are.' % z.support)
+        self.latent_vars_unconstrained[z] = qz
+    self.logging = logdir is not None
+    if self.logging:
+      if log_timestamp:
+        logdir = os.path.join(os.path.expanduser(logdir),
+                              time.strftime('%Y%m%d_%H%M%S', time.gmtime()))
+      os.makedirs(logdir, exist_ok=True)
+      self._log_path = os.path.join(logdir, 'metrics.jsonl')
+    self.debug = debug
+    self._initialized = True
+    return self
+
+  def _initialize_variables(self, variables):
+    pass
+
+  def update(self, feed_dict=None):
+    raise NotImplementedError()
+
+  def print_progress(self, info_dict):
+    raise NotImplementedError()
+
+  def finalize(self):
+    """``inference.py:334-338``."""
+    if getattr(self, 'logging', False):
+      pass
+
+  def _log(self, rec):
+    if getattr(self, 'logging', False):
+      with open(self._log_path, 'a') as f:
+        f.write(json.dumps(rec) + '\n')
+
+
+class MonteCarlo(Inference):
+  """``edward/inferences/monte_carlo.py:15-170``."""
+
+  def __init__(self, latent_vars=None, data=None):
+    if isinstance(latent_vars, list):
+      latent_vars = {z: M.Empirical(params=torch.zeros(
+          (10000,) + tuple(z.shape), dtype=getattr(torch, z.dtype)))
+          for z in latent_vars}
+    elif isinstance(latent_vars, dict):
+      for qz in latent_vars.values():
+        if not isinstance(qz, M.Empirical):
+          raise TypeError('Posterior approximation must consist of only Empirical '
+                          'random variables.')
+        elif len(qz.sample_shape) != 0:
+          raise ValueError('Empirical posterior approximations must have a scalar '
+                           'sample shape.')
+    super(MonteCarlo, self).__init__(latent_vars, data)
+
+  def initialize(self, *args, **kwargs):
+    kwargs['n_iter'] = int(np.amin([qz.params.shape[0] for qz in self.latent_vars.values()]))
+    super(MonteCarlo, self).initialize(*args, **kwargs)
+    self.train = self.build_update()
+    self._reset_counts()
+    return self
+
+  def _reset_counts(self):
+    self.t = 0
+    self._n_accept = torch.zeros(self.n_chains, dtype=torch.int32, device=self._device)
+
+  @property
+  def n_accept(self):
+    """Accepted transitions: an int for one chain, an int array over chains."""
+    a = self._n_accept.cpu().numpy()
+    return int(a[0]) if self.n_chains == 1 else a
+
+  def reset(self):
+    """The reference's ``reset`` ops (``inference.py:285``,
+    ``monte_carlo.py:104``): t = 0 and n_accept = 0."""
+    self._reset_counts()
+    self._state_t = None
+
+  def update(self, feed_dict=None):
+    """``monte_carlo.py:111-150``: one transition, then t += 1."""
+    self._step()
+    self.t += 1
+    acc = self._n_accept.to(torch.float64).mean().item() / self.t
+    info = {'t': self.t, 'accept_rate': acc}
+    if self.debug:
+      nd = int(self.engine.divergences().sum().item())
+      if nd:
+        raise FloatingPointError('%d chain(s) produced non-finite proposals' % nd)
+    if self.logging and self.n_print != 0 and (self.t == 1 or self.t % self.n_print == 0):
+      self._log(info)
+    return info
+
+  def print_progress(self, info_dict):
+    """``monte_carlo.py:152-158``."""
+    if self.n_print != 0:
+      t = info_dict['t']
+      if t == 1 or t % self.n_print == 0:
+        self.progbar.update(t, {'Acceptance Rate': info_dict['accept_rate']})
+
+  def build_update(self):
+    raise NotImplementedError()
+
+
+class HMC(MonteCarlo):
+  """Hamiltonian Monte Carlo (``edward/inferences/hmc.py:14-130``).
+
+  ``HMC(latent_vars={w: Empirical(params)}, data={X: X_data, y: y_data})``;
+  ``initialize(step_size=0.25, n_steps=2, ...)``; ``run()`` or ``update()``.
+  Build-only keyword arguments of ``initialize``:
+    precision  'fp32' (default) or 'bf16' (X in bf16 on the matrix cores)
+    seed       Philox key of the device momenta/uniforms (default: the
+               ``ed.set_seed`` value, else 0)
+    surrogate  a trained ``edward_amd.surrogate.Surrogate`` to drive the
+               leapfrog (the exact log joint still decides acceptance)
+  """
+
+  def __init__(self, *args, **kwargs):
+    super(HMC, self).__init__(*args, **kwargs)
+
+  def initialize(self, step_size=0.25, n_steps=2, *args, **kwargs):
+    """``hmc.py:45-59``; returns ``self``."""
+    self.step_size = step_size
+    self.n_steps = int(n_steps)
+    self.precision = kwargs.pop('precision', 'fp32')
+    from edward_amd import util
+    seed = kwargs.pop('seed', None)
+    self.seed = util.get_seed() if seed is None else int(seed)
+    self.surrogate = kwargs.pop('surrogate', None)
+    return super(HMC, self).initialize(*args, **kwargs)
+
+  def build_update(self):
+    """Compile the model (``hmc.py:61-130`` builds the transition graph)."""
+    if not hasattr(self, 'latent_vars_unconstrained'):
+      raise ValueError('This implementation of HMC requires that all variables have '
+                       'unconstrained support. Please initialize with auto_transform=True '
+                       'to ensure this. (if your variables already have unconstrained '
+                       'support then doing this is a no-op).')
+    prog = _compile.compile_model(self.latent_vars, self.data, precision=self.precision)
+    self.program = prog
+    self.engine = prog.engine
+    self.n_chains = prog.n_chains
+    self._device = self.engine.device
+    self._state_t = None
+    self._row = torch.empty(prog.n_chains, prog.dim, device=self._device)
+    if self.surrogate is not None:
+      self.surrogate.attach(self.engine)
+    return self
+
+  def _step(self):
+    t = self.t
+    T = self.program.n_rows
+    if t >= T:
+      raise IndexError('t=%d is past the %d rows of the Empirical store' % (t, T))
+    need = max(t - 1, 0)
+    if self._state_t != need:
+      self.engine.set_state(self.program.gather(need))
+    if self.surrogate is None:
+      self.engine.transition(self.step_size, self.n_steps, seed=self.seed, counter=t,
+                             sample_out=self._row, n_accept=self._n_accept)
+    else:
+      self.surrogate.transition(self.step_size, self.n_steps, seed=self.seed, counter=t,
+                                sample_out=self._row, n_accept=self._n_accept)
+    self.program.scatter(t, self._row)
+    self._state_t = t
+
+  @property
+  def divergences(self):
+    return self.engine.divergences().cpu().numpy()

@@ -1,0 +1,213 @@
+/*
+ * edhmc -- MI355X-native Hamiltonian Monte Carlo engine, C ABI.
+ *
+ * This is the drop-in boundary under Edward's `ed.HMC` surface.  In Edward
+ * 1.3.5 the hot path is one TensorFlow session call per sample:
+ * `MonteCarlo.update` (edward/inferences/monte_carlo.py:111-150) runs the op
+ * that `HMC.build_update` (edward/inferences/hmc.py:61-130) built, whose
+ * arithmetic is `leapfrog` (hmc.py:195-210) over `HMC._log_joint`
+ * (hmc.py:161-192).  Each entry point below replaces one of those pieces;
+ * its comment cites the reference lines whose behaviour it reproduces.
+ *
+ * Conventions
+ *   - All array arguments are DEVICE pointers on the context's GPU, owned by
+ *     the caller; the library never frees them.  The library owns only the
+ *     scratch behind an opaque context (data layouts, residuals, partial
+ *     sums, chain state).
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *     Calls enqueue work and return; the caller synchronises.
+ *   - Every function returns 0 on success or a negative EDHMC_E* code;
+ *     edhmc_last_error() then holds a message.  No C++ exception crosses
+ *     the ABI.  A context is not thread-safe (one per thread or a lock).
+ *   - Matrices are row-major.  C chains of a D-dimensional latent are [C, D].
+ */
+#ifndef EDHMC_H_
+#define EDHMC_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct edhmc_ctx edhmc_ctx;
+
+#define EDHMC_OK 0
+#define EDHMC_EINVAL (-1)      /* bad argument (TypeError/ValueError in Edward) */
+#define EDHMC_ENOMEM (-2)      /* device allocation failed */
+#define EDHMC_EHIP (-3)        /* HIP runtime error */
+#define EDHMC_EUNSUPPORTED (-4)
+#define EDHMC_ESTATE (-5)      /* call order: e.g. transition before bind */
+#define EDHMC_ENONFINITE (-6)  /* non-finite data (InvalidArgumentError in ed.dot) */
+
+#define EDHMC_F32 0
+#define EDHMC_BF16 1
+
+#define EDHMC_LIK_BERNOULLI 0  /* Bernoulli(logits = X w)          */
+#define EDHMC_LIK_NORMAL 1     /* Normal(loc = X w, scale = s)      */
+
+/* Library version string. */
+const char* edhmc_version(void);
+
+/* Create a context on HIP device `device`.  Replaces the global session of
+ * edward/util/graphs.py:13-41 (get_session). */
+int edhmc_create(int device, edhmc_ctx** out);
+int edhmc_destroy(edhmc_ctx* ctx);
+const char* edhmc_last_error(const edhmc_ctx* ctx);
+
+/* Bind a generalised linear model  y ~ Lik(dot(X, w)),  w ~ Normal(loc, scale).
+ * Replaces the data binding of edward/inferences/inference.py:81-95 (data
+ * copied once into a runtime-owned variable) and the model graph whose
+ * log-joint hmc.py:161-192 sums: prior Normal.log_prob + likelihood log_prob
+ * (Bernoulli(logits=ed.dot(X, w)), examples/bayesian_logistic_regression.py:41-44,
+ * or Normal(loc=ed.dot(X, w), scale=s), tests/inferences/hmc_test.py:63-66).
+ *   X         [N, D] in x_in_dtype (EDHMC_F32 or EDHMC_BF16)
+ *   y         [N] float32 (0/1 for Bernoulli)
+ *   prior_loc, prior_scale  [D] float32
+ *   compute_dtype  EDHMC_F32: fp32 X, fp32 MFMA;  EDHMC_BF16: X rounded to
+ *                  bf16 (RNE), bf16 MFMA with fp32 accumulation.
+ * Non-finite X raises EDHMC_ENONFINITE like ed.dot's verify_tensor_all_finite
+ * (edward/util/tensorflow.py:33-36).  Rebinding discards chain state. */
+int edhmc_bind_glm(edhmc_ctx* ctx, const void* X, int32_t x_in_dtype, const float* y,
+                   int64_t N, int32_t D, int32_t likelihood, float lik_scale,
+                   const float* prior_loc, const float* prior_scale,
+                   int32_t compute_dtype, void* stream);
+
+/* Set the C chains' positions W [C, D] and evaluate logp and its gradient
+ * there (the state every following transition starts from).  This is the
+ * gather `params[max(t-1, 0)]` of hmc.py:81-85 when a run starts.
+ * Non-finite W -> EDHMC_ENONFINITE (ed.dot check). */
+int edhmc_set_state(edhmc_ctx* ctx, const float* W, int32_t C, void* stream);
+
+/* Copy the current positions [C, D], log joints [C] (fp64) and gradients
+ * [C, D]; any output may be NULL. */
+int edhmc_get_state(edhmc_ctx* ctx, float* W_out, double* logp_out, float* grad_out,
+                    void* stream);
+
+/* Log joint and gradient at W [C, D]: logp_out [C] fp64, grad_out [C, D].
+ * The value of HMC._log_joint_unconstrained (hmc.py:132-159, identity
+ * transform for real support) and of tf.gradients of it (hmc.py:199,206).
+ * Also makes W the current chain state.  Parity hook. */
+int edhmc_logp_grad(edhmc_ctx* ctx, const float* W, int32_t C, float* grad_out,
+                    double* logp_out, void* stream);
+
+/* One HMC transition of every chain: momentum draw (hmc.py:88-91),
+ * leapfrog with n_steps steps of size eps (hmc.py:94-97,195-210), ratio and
+ * accept (hmc.py:100-111), selection, and the sample write + n_accept
+ * increment (hmc.py:121-129).
+ *   r0       [C, D] fp32 momenta, or NULL to draw them on the device
+ *            (Philox4x32-10, key = seed, counter = `counter`)
+ *   u        [C] fp64 accept uniforms, or NULL to draw them on the device
+ *   accepted_out [C] int32 0/1 (may be NULL)
+ *   sample_out   [C, D] fp32: the Empirical row params[t] (may be NULL)
+ *   n_accept     [C] int32, incremented on accept (may be NULL)
+ * The gradient at the current state is carried from the previous
+ * transition (the reference re-evaluates it, hmc.py:199; the value is the
+ * same), so one transition costs n_steps gradient evaluations. */
+int edhmc_transition(edhmc_ctx* ctx, float eps, int32_t n_steps, const float* r0,
+                     const double* u, uint64_t seed, uint64_t counter,
+                     int32_t* accepted_out, float* sample_out, int32_t* n_accept,
+                     void* stream);
+
+/* The run loop of Inference.run (edward/inferences/inference.py:145-147):
+ * transitions t = t0 .. t0+n_iter-1, each writing params[t] of the sample
+ * store params [T, C, D] and using momentum counter counter0 + t.  The chain
+ * state must already equal params[max(t0-1, 0)] (edhmc_set_state). */
+int edhmc_run(edhmc_ctx* ctx, float eps, int32_t n_steps, float* params, int64_t T,
+              int64_t t0, int64_t n_iter, uint64_t seed, uint64_t counter0,
+              int32_t* n_accept, void* stream);
+
+/* Running FP64 per-chain moments (Welford) of every sample the transitions
+ * write; the readout of Empirical.mean/stddev (edward/models/empirical.py:87-96)
+ * without reading the whole sample store.  enable != 0 clears them. */
+int edhmc_moments_enable(edhmc_ctx* ctx, int32_t enable);
+/* mean_out, m2_out [C, D] fp64 (m2 = sum of squared deviations); count_out
+ * host int64. */
+int edhmc_moments_get(edhmc_ctx* ctx, double* mean_out, double* m2_out,
+                      int64_t* count_out, void* stream);
+
+/* Number of chains whose proposal was non-finite (rejected) since binding,
+ * [C] int32 device array copy. */
+int edhmc_divergences(edhmc_ctx* ctx, int32_t* out, void* stream);
+
+/* Device RNG parity hooks: the momenta / uniforms edhmc_transition would
+ * draw for (seed, counter).  r_out [C, D] fp32, u_out [C] fp64. */
+int edhmc_draw_momenta(edhmc_ctx* ctx, int32_t C, int32_t D, uint64_t seed,
+                       uint64_t counter, float* r_out, void* stream);
+int edhmc_draw_uniforms(edhmc_ctx* ctx, int32_t C, uint64_t seed, uint64_t counter,
+                        double* u_out, void* stream);
+
+/* Multi-chain autocovariance for effective-sample-size estimation over a
+ * sample store s [T, C, D] fp32 (no reference counterpart; build-only):
+ *   acov_out   [max_lag+1, D] fp64  sum over chains of the biased (/T)
+ *              within-chain autocovariance at each lag
+ *   chain_mean [C, D] fp64 (may be NULL to skip)                            */
+int edhmc_autocov(edhmc_ctx* ctx, const float* s, int64_t T, int32_t C, int32_t D,
+                  int32_t max_lag, double* acov_out, double* chain_mean_out,
+                  void* stream);
+
+/* ------------------------------------------------------------------------
+ * Learned gradient surrogate (no reference counterpart; build-only).
+ * An MLP D -> H -> H -> D with tanh-GELU, fp32 master weights, bf16 MFMA
+ * GEMMs with fp32 accumulation, trained with Adam on (w, grad log p) pairs.
+ * It drives the inner leapfrog steps of edhmc_transition_surrogate; the exact
+ * log joint still decides every Metropolis test.
+ * Weight layout (fp32, one flat buffer, PyTorch Linear convention):
+ *   W1 [H, D] b1 [H] W2 [H, H] b2 [H] W3 [D, H] b3 [D]
+ * Normalisation: x = (w - in_shift) * in_scale,  grad = out * out_scale + out_shift.
+ * ------------------------------------------------------------------------ */
+int edhmc_sur_create(edhmc_ctx* ctx, int32_t D, int32_t H, int32_t max_batch);
+int64_t edhmc_sur_num_params(edhmc_ctx* ctx);
+int edhmc_sur_set_params(edhmc_ctx* ctx, const float* params, void* stream);
+int edhmc_sur_get_params(edhmc_ctx* ctx, float* params_out, void* stream);
+int edhmc_sur_set_norm(edhmc_ctx* ctx, const float* in_shift, const float* in_scale,
+                       const float* out_shift, const float* out_scale, void* stream);
+/* grad_out [B, D] = surrogate gradient at W [B, D]. */
+int edhmc_sur_forward(edhmc_ctx* ctx, const float* W, int32_t B, float* grad_out,
+                      void* stream);
+/* Loss = mean over batch and coordinates of (out - target_n)^2 with
+ * target_n = (G - out_shift) / out_scale.  Writes the loss (fp64, device
+ * [1]) and the gradient of the loss w.r.t. the flat parameters (fp32, device,
+ * edhmc_sur_num_params entries).  Parity hook for the backward kernels. */
+int edhmc_sur_loss_grad(edhmc_ctx* ctx, const float* W, const float* G, int32_t B,
+                        double* loss_out, float* param_grad_out, void* stream);
+/* One Adam step (Kingma & Ba 2015) on a batch: loss_grad then
+ * m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2,
+ * p -= lr (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + adam_eps). */
+int edhmc_sur_train_step(edhmc_ctx* ctx, const float* W, const float* G, int32_t B,
+                         float lr, float beta1, float beta2, float adam_eps, int32_t step,
+                         double* loss_out, void* stream);
+/* HMC transition whose leapfrog forces come from the surrogate; the ratio
+ * uses the exact log joint at the end point (bf16 X with a hi/lo bf16 split
+ * of w, FP64 reduction), so the chain targets the exact posterior.  Same
+ * arguments and semantics as edhmc_transition. */
+int edhmc_transition_surrogate(edhmc_ctx* ctx, float eps, int32_t n_steps, const float* r0,
+                               const double* u, uint64_t seed, uint64_t counter,
+                               int32_t* accepted_out, float* sample_out, int32_t* n_accept,
+                               void* stream);
+
+/* ------------------------------------------------------------------------
+ * Batched hierarchical logistic regression (G groups, rows sorted by group):
+ *   a_g ~ N(mu, exp(log_sigma)^2), mu ~ N(0,1), log_sigma ~ N(0,1),
+ *   w ~ N(0, I_D), y_n ~ Bernoulli(a_{g(n)} + x_n . w).
+ * Latent layout per chain: [a (G), mu, log_sigma, w (D)].  In Edward:
+ * Bernoulli(logits=tf.gather(a, group) + ed.dot(X, w)); the gather gradient
+ * is the IndexedSlices case of tests/inferences/hmc_test.py:105-123 and
+ * hmc.py:203,208.  One persistent cooperative launch runs all n_steps.
+ * ------------------------------------------------------------------------ */
+int edhmc_bind_hier(edhmc_ctx* ctx, const float* X, const float* y, const int64_t* offsets,
+                    int32_t G, int64_t N, int32_t D, void* stream);
+int edhmc_hier_logp_grad(edhmc_ctx* ctx, const float* theta, int32_t C, float* grad_out,
+                         double* logp_out, void* stream);
+int edhmc_hier_set_state(edhmc_ctx* ctx, const float* theta, int32_t C, void* stream);
+int edhmc_hier_transition(edhmc_ctx* ctx, float eps, int32_t n_steps, const float* r0,
+                          const double* u, uint64_t seed, uint64_t counter,
+                          int32_t* accepted_out, float* sample_out, int32_t* n_accept,
+                          void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EDHMC_H_ */
