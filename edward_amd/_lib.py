@@ -51,6 +51,9 @@ SIGNATURES = {
     'edhmc_moments_enable': (ctypes.c_int, [_c_p, _i32]),
     'edhmc_moments_get': (ctypes.c_int, [_c_p, _c_p, _c_p, ctypes.POINTER(_i64), _c_p]),
     'edhmc_divergences': (ctypes.c_int, [_c_p, _c_p, _c_p]),
+    'edhmc_profile_enable': (ctypes.c_int, [_c_p, _i32]),
+    'edhmc_profile_read': (ctypes.c_int, [_c_p, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(_i64)]),
     'edhmc_draw_momenta': (ctypes.c_int, [_c_p, _i32, _i32, _u64, _u64, _c_p, _c_p]),
     'edhmc_draw_uniforms': (ctypes.c_int, [_c_p, _i32, _u64, _u64, _c_p, _c_p]),
     'edhmc_autocov': (ctypes.c_int, [_c_p, _c_p, _i64, _i32, _i32, _i32, _c_p, _c_p, _c_p]),

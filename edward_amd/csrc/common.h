@@ -74,10 +74,23 @@ struct LikParams {
   double ll_const; // -log s - log(2 pi)/2 (normal)
 };
 
+// The epilogues call only native transcendentals (v_exp_f32, v_log_f32,
+// v_rcp_f32): an out-of-line math call inside an epilogue would force the
+// live accumulators to scratch memory.
+__device__ inline float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// log(1 + e) for e in [0, 1]: u = 1 + e rounds, and log(u) * e / (u - 1)
+// cancels that rounding (u - 1 is exact); e below half an ulp of 1 gives e.
+__device__ inline float log1p_unit(float e) {
+  const float u = 1.0f + e;
+  const float l = __logf(u);
+  return u == 1.0f ? e : l * (e * fast_rcp(u - 1.0f));
+}
+
 __device__ inline float glm_dll(const LikParams& p, float eta, float y) {
   if (p.lik == LIK_BERNOULLI) {
     float e = __expf(-fabsf(eta));
-    float inv = __frcp_rn(1.0f + e);
+    float inv = fast_rcp(1.0f + e);
     float sig = eta >= 0.0f ? inv : e * inv;
     return y - sig;
   }
@@ -87,7 +100,7 @@ __device__ inline float glm_dll(const LikParams& p, float eta, float y) {
 __device__ inline double glm_ll(const LikParams& p, float eta, float y) {
   if (p.lik == LIK_BERNOULLI) {
     float e = __expf(-fabsf(eta));
-    float sp = fmaxf(eta, 0.0f) + log1pf(e);
+    float sp = fmaxf(eta, 0.0f) + log1p_unit(e);
     return (double)(y * eta) - (double)sp;
   }
   float d = y - eta;
@@ -98,10 +111,10 @@ __device__ inline void glm_both(const LikParams& p, float eta, float y,
                                 float& dll, double& ll) {
   if (p.lik == LIK_BERNOULLI) {
     float e = __expf(-fabsf(eta));
-    float inv = __frcp_rn(1.0f + e);
+    float inv = fast_rcp(1.0f + e);
     float sig = eta >= 0.0f ? inv : e * inv;
     dll = y - sig;
-    float sp = fmaxf(eta, 0.0f) + log1pf(e);
+    float sp = fmaxf(eta, 0.0f) + log1p_unit(e);
     ll = (double)(y * eta) - (double)sp;
   } else {
     float d = y - eta;

@@ -17,6 +17,36 @@ namespace edhmc {
 struct Surrogate;   // mlp.hip
 struct HierModel;   // hier.hip
 
+// Kernel classes timed by the event profiler.
+enum ProfKind {
+  PK_GEMM_GRAD = 0,   // GEMM1 with residual epilogue (gradient pass)
+  PK_GEMM_LL = 1,     // GEMM1 forward with log-likelihood epilogue only
+  PK_GEMM_XTR = 2,    // GEMM2 (X^T residual, split-K)
+  PK_CHAIN = 3,       // per-chain leapfrog / accept kernels
+  PK_MLP = 4,         // surrogate MLP GEMMs (+ fused epilogues)
+  PK_HIER = 5,        // hierarchical persistent kernel
+  PK_COUNT = 6
+};
+
+// hipEvent pairs recorded around launches on the launch stream; summed at
+// collection time.  Off unless enabled.
+struct Profiler {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<int, int>> marks;   // (kind, index of start event)
+  double total_ms[PK_COUNT] = {};
+  int64_t count[PK_COUNT] = {};
+  size_t next = 0;
+  hipEvent_t get() {
+    if (next == pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      pool.push_back(e);
+    }
+    return pool[next++];
+  }
+};
+
 // A device allocation owned by the context.
 struct DevBuf {
   void* p = nullptr;
@@ -48,7 +78,7 @@ struct edhmc_ctx {
   bool state_valid = false;
   edhmc::DevBuf z, r, g, z0, g0, z_lp, logp, K0, flag, ndiv;
   // ---- gradient scratch ----
-  edhmc::DevBuf Rt, Gpart, llpart;
+  edhmc::DevBuf Rt, Gpart, llpart, ll;
   int ksplit = 1;
   int n_tiles1 = 0;      // GEMM1 column tiles (= llpart leading dim)
   // ---- moments ----
@@ -58,6 +88,7 @@ struct edhmc_ctx {
   // ---- misc scratch ----
   edhmc::DevBuf tmp_count;
 
+  edhmc::Profiler prof;
   edhmc::Surrogate* sur = nullptr;
   edhmc::HierModel* hier = nullptr;
 
@@ -94,6 +125,18 @@ inline int dev_alloc(edhmc_ctx* ctx, DevBuf& b, size_t bytes, bool zero = true) 
     return set_err(ctx, EDHMC_EHIP, "hipMemset failed");
   return EDHMC_OK;
 }
+
+// Bracket a launch with profiler events when profiling is on.
+struct ProfScope {
+  edhmc_ctx* ctx;
+  hipStream_t st;
+  int kind;
+  int idx = -1;
+  ProfScope(edhmc_ctx* c, int k, hipStream_t s);
+  ~ProfScope();
+};
+
+int prof_collect(edhmc_ctx* ctx);   // sync + fold recorded pairs into totals
 
 inline int check_launch(edhmc_ctx* ctx, const char* what) {
   hipError_t e = hipGetLastError();

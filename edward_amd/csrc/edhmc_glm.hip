@@ -21,8 +21,42 @@
 
 namespace edhmc {
 
+ProfScope::ProfScope(edhmc_ctx* c, int k, hipStream_t s) : ctx(c), st(s), kind(k) {
+  if (!ctx->prof.on) return;
+  if (ctx->prof.marks.size() >= 4096) prof_collect(ctx);
+  hipEvent_t e = ctx->prof.get();
+  if (!e) return;
+  idx = (int)ctx->prof.next - 1;
+  (void)hipEventRecord(e, st);
+}
+
+ProfScope::~ProfScope() {
+  if (idx < 0) return;
+  hipEvent_t e = ctx->prof.get();
+  if (!e) return;
+  (void)hipEventRecord(e, st);
+  ctx->prof.marks.push_back({kind, idx});
+}
+
+int prof_collect(edhmc_ctx* ctx) {
+  Profiler& p = ctx->prof;
+  for (auto& m : p.marks) {
+    hipEvent_t a = p.pool[m.second], b = p.pool[m.second + 1];
+    if (hipEventSynchronize(b) != hipSuccess) return set_err(ctx, EDHMC_EHIP, "event sync");
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) == hipSuccess) {
+      p.total_ms[m.first] += ms;
+      p.count[m.first] += 1;
+    }
+  }
+  p.marks.clear();
+  p.next = 0;
+  return EDHMC_OK;
+}
+
 using Cfg1 = TileCfg<2, 2, 2, 2>;   // 128 x 128, GEMM1 and wide GEMM2
 using Cfg2S = TileCfg<1, 1, 4, 1>;  // 128 x 32, GEMM2 for D_pad % 128 != 0
+using CfgP2 = TileCfg<1, 2, 2, 2>;  // 64 x 128, GEMM1 over hi/lo split positions
 constexpr int kChainThreads = 256;
 
 // ---------------------------------------------------------------------------
@@ -34,64 +68,61 @@ struct EpiGLM {
   const float* y;
   int64_t N;
   LikParams lp;
-  T* Rt;              // [C_pad, N_pad]
-  int64_t ldr;
-  double* llpart;     // [C_pad, n_tiles]
-  int64_t ll_ld;
+  T* Rt;              // K-blocked [N_pad / KE][C_pad][KE], KE = 64 bytes of T
+  int64_t ldr;        // C_pad
+  double* llpart;     // [n_tiles][C_pad]
+  int64_t ll_ld;      // C_pad
 
   template <typename Cfg>
   __device__ void apply(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p) const {
     constexpr int WM = Cfg::kWM, WN = Cfg::kWN;
-    double lls[WM][16];
-    if (LL) {
-#pragma unroll
-      for (int i = 0; i < WM; ++i)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) lls[i][r] = 0.0;
-    }
+    constexpr int BM = Cfg::BM;
+    constexpr int KE = kStageBytes / sizeof(T);
+    double* sm = reinterpret_cast<double*>(p.smem);
+    float yv[WN];
+    bool valid[WN];
 #pragma unroll
     for (int j = 0; j < WN; ++j) {
       const int64_t n = p.n0 + 32 * j + (p.lane & 31);
-      const bool valid = n < N;
-      const float yv = valid ? y[n] : 0.0f;
+      valid[j] = n < N;
+      yv[j] = valid[j] ? y[n] : 0.0f;
+    }
 #pragma unroll
-      for (int i = 0; i < WM; ++i)
+    for (int i = 0; i < WM; ++i)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
+      for (int r = 0; r < 16; ++r) {
+        const int64_t m = p.m0 + 32 * i + acc_row(r, p.lane);
+        double llsum = 0.0;
+#pragma unroll
+        for (int j = 0; j < WN; ++j) {
+          const int64_t n = p.n0 + 32 * j + (p.lane & 31);
           const float eta = acc[i][j][r];
           float dll;
-          double ll;
           if (LL) {
-            glm_both(lp, eta, yv, dll, ll);
-            lls[i][r] += valid ? ll : 0.0;
+            double ll;
+            glm_both(lp, eta, yv[j], dll, ll);
+            llsum += valid[j] ? ll : 0.0;
           } else {
-            dll = glm_dll(lp, eta, yv);
+            dll = glm_dll(lp, eta, yv[j]);
           }
-          if (WR) {
-            const int64_t m = p.m0 + 32 * i + acc_row(r, p.lane);
-            Rt[m * ldr + n] = ElemOps<T>::from_f32(valid ? dll : 0.0f);
-          }
+          if (WR)
+            Rt[((n / KE) * ldr + m) * KE + (n % KE)] =
+                ElemOps<T>::from_f32(valid[j] ? dll : 0.0f);
         }
-    }
-    if (LL) {
-      constexpr int BM = Cfg::BM;
-      double* sm = reinterpret_cast<double*>(p.smem);
+        if (LL) {
 #pragma unroll
-      for (int i = 0; i < WM; ++i)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          double v = lls[i][r];
-#pragma unroll
-          for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+          for (int off = 16; off > 0; off >>= 1) llsum += __shfl_xor(llsum, off, 64);
           if ((p.lane & 31) == 0)
-            sm[p.wn * BM + p.wm * 32 * WM + 32 * i + acc_row(r, p.lane)] = v;
+            sm[p.wn * BM + p.wm * 32 * WM + 32 * i + acc_row(r, p.lane)] = llsum;
         }
+      }
+    if (LL) {
       __syncthreads();
       for (int t = threadIdx.x; t < BM; t += Cfg::kThreads) {
         double s = 0.0;
 #pragma unroll
         for (int w = 0; w < Cfg::kWavesN; ++w) s += sm[w * BM + t];
-        llpart[(int64_t)(p.tm * BM + t) * ll_ld + p.tn] = s;
+        llpart[(int64_t)p.tn * ll_ld + p.tm * BM + t] = s;
       }
     }
   }
@@ -124,11 +155,27 @@ layout_x_kernel(const Tin* __restrict__ X, int64_t N, int D, T* __restrict__ Xr,
   }
   if (bad) atomicAdd(nonfinite, 1);
   __syncthreads();
+  // Xc is K-blocked: [N_pad / KE][D_pad][KE] with KE = 64 bytes of T.
+  constexpr int KE = kStageBytes / sizeof(T);
   for (int k = ty; k < 32; k += 8) {
     const int d = d0 + k;
     const int64_t n = n0 + tx;
-    Xc[(int64_t)d * N_pad + n] = ElemOps<T>::from_f32(tile[tx][k]);
+    Xc[((n / KE) * D_pad + d) * KE + (n % KE)] = ElemOps<T>::from_f32(tile[tx][k]);
   }
+}
+
+// Deterministic reduction of the log-likelihood partials llpart[t][c] over
+// the row tiles t: 64 chains per block, 4 fixed strides of t, fixed order.
+__global__ void __launch_bounds__(256)
+ll_reduce_kernel(const double* __restrict__ llpart, int n_tiles, int C_pad, double* __restrict__ ll) {
+  __shared__ double part[4][64];
+  const int tc = threadIdx.x & 63, tg = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + tc;
+  double s = 0.0;
+  for (int t = tg; t < n_tiles; t += 4) s += llpart[(int64_t)t * C_pad + c];
+  part[tg][tc] = s;
+  __syncthreads();
+  if (tg == 0) ll[c] = ((part[0][tc] + part[1][tc]) + part[2][tc]) + part[3][tc];
 }
 
 __global__ void pad_copy_kernel(const float* __restrict__ src, int64_t n, float* __restrict__ dst,
@@ -157,7 +204,7 @@ struct ChainArgs {
   const float* r_fed; const double* u_fed;
   uint64_t seed, counter;
   const float* Gpart; int ksplit;
-  const double* llpart; int ll_ld; int n_ll;
+  const double* ll;         // [C_pad] reduced log-likelihood
   float* sample_out; int32_t* accepted_out; int32_t* n_accept; int32_t* ndiv;
   double* mom_mean; double* mom_m2; int64_t mom_n;   // mom_n = count after update
   float* grad_out; double* logp_out;
@@ -234,12 +281,6 @@ __device__ inline float gsum(const ChainArgs& a, int c, int d) {
   return s;
 }
 
-__device__ inline double ll_total(const ChainArgs& a, int c, double* red) {
-  double s = 0.0;
-  const double* p = a.llpart + (int64_t)c * a.ll_ld;
-  for (int t = threadIdx.x; t < a.n_ll; t += kChainThreads) s += p[t];
-  return block_sum_f64<kChainThreads>(s, red);
-}
 
 // Middle of the trajectory: gradient at the new z, r += eps/2 g twice
 // (end of this step, start of the next), drift.
@@ -282,7 +323,7 @@ __device__ inline void chain_accept(const ChainArgs& a, GradAt g_at) {
   }
   kl = block_sum_f64<kChainThreads>(kl, red);
   plp = block_sum_f64<kChainThreads>(plp, red);
-  const double ll = ll_total(a, c, red);
+  const double ll = a.ll[c];
   const double nbad = block_sum_f64<kChainThreads>((double)bad, red);
   if (threadIdx.x == 0) {
     const double logpL = ll + plp;
@@ -340,7 +381,7 @@ __global__ void __launch_bounds__(kChainThreads) chain_eval_kernel(ChainArgs a) 
     plp += prior_logp(a, d, zv);
   }
   plp = block_sum_f64<kChainThreads>(plp, red);
-  const double ll = ll_total(a, c, red);
+  const double ll = a.ll[c];
   if (threadIdx.x == 0) {
     a.logp[c] = ll + plp;
     if (a.logp_out) a.logp_out[c] = ll + plp;
@@ -462,7 +503,7 @@ static ChainArgs chain_args(edhmc_ctx* ctx) {
   a.logp = (double*)ctx->logp.p; a.K0 = (double*)ctx->K0.p;
   a.ploc = (const float*)ctx->ploc.p; a.pscale = (const float*)ctx->pscale.p;
   a.Gpart = (const float*)ctx->Gpart.p; a.ksplit = ctx->ksplit;
-  a.llpart = (const double*)ctx->llpart.p; a.ll_ld = ctx->n_tiles1; a.n_ll = ctx->n_tiles1;
+  a.ll = (const double*)ctx->ll.p;
   a.ndiv = (int32_t*)ctx->ndiv.p;
   return a;
 }
@@ -473,26 +514,35 @@ static int launch_gemm1(edhmc_ctx* ctx, hipStream_t st) {
   GemmShape g = {};
   g.A = (const char*)ctx->z_lp.p;
   g.lda_b = (int64_t)ctx->D_pad * eb;
+  g.a_stage_b = kStageBytes;
   g.a_plane_b = (int64_t)ctx->C_pad * ctx->D_pad * eb;
   g.B = (const char*)ctx->Xr.p;
   g.ldb_b = (int64_t)ctx->D_pad * eb;
+  g.b_stage_b = kStageBytes;
   g.m_tiles = ctx->C_pad / Cfg::BM;
   g.n_tiles = (int)(ctx->N_pad / Cfg::BN);
   g.k_stages = ctx->D_pad * eb / kStageBytes;
   g.ksplit = 1;
   g.stages_per_split = g.k_stages;
   g.swz = 1;
+  ProfScope ps(ctx, WR ? PK_GEMM_GRAD : PK_GEMM_LL, st);
   EpiGLM<T, WR, LL> e;
   e.y = (const float*)ctx->y.p;
   e.N = ctx->N;
   e.lp = lik_params(ctx);
   e.Rt = (T*)ctx->Rt.p;
-  e.ldr = ctx->N_pad;
+  e.ldr = ctx->C_pad;
   e.llpart = (double*)ctx->llpart.p;
-  e.ll_ld = ctx->n_tiles1;
+  e.ll_ld = ctx->C_pad;
   hipLaunchKernelGGL((gemm_nt_kernel<T, Cfg, PLANES, EpiGLM<T, WR, LL>>), dim3(grid_size(g)),
                      dim3(Cfg::kThreads), 0, st, g, e);
-  return check_launch(ctx, "gemm1");
+  EDHMC_TRY(check_launch(ctx, "gemm1"));
+  if (LL) {
+    hipLaunchKernelGGL(ll_reduce_kernel, dim3(ctx->C_pad / 64), dim3(256), 0, st,
+                       (const double*)ctx->llpart.p, ctx->n_tiles1, ctx->C_pad, (double*)ctx->ll.p);
+    EDHMC_TRY(check_launch(ctx, "ll_reduce"));
+  }
+  return EDHMC_OK;
 }
 
 template <typename T, typename Cfg>
@@ -500,16 +550,19 @@ static int launch_gemm2_cfg(edhmc_ctx* ctx, hipStream_t st) {
   const int eb = sizeof(T);
   GemmShape g = {};
   g.A = (const char*)ctx->Rt.p;
-  g.lda_b = ctx->N_pad * eb;
+  g.lda_b = kStageBytes;
+  g.a_stage_b = (int64_t)ctx->C_pad * kStageBytes;
   g.a_plane_b = 0;
   g.B = (const char*)ctx->Xc.p;
-  g.ldb_b = ctx->N_pad * eb;
+  g.ldb_b = kStageBytes;
+  g.b_stage_b = (int64_t)ctx->D_pad * kStageBytes;
   g.m_tiles = ctx->C_pad / Cfg::BM;
   g.n_tiles = ctx->D_pad / Cfg::BN;
   g.k_stages = (int)(ctx->N_pad * eb / kStageBytes);
   g.ksplit = ctx->ksplit;
   g.stages_per_split = (g.k_stages + g.ksplit - 1) / g.ksplit;
   g.swz = (g.ksplit % 8 == 0) ? 2 : 0;
+  ProfScope ps(ctx, PK_GEMM_XTR, st);
   EpiStoreF32 e;
   e.out = (float*)ctx->Gpart.p;
   e.ldc = ctx->D_pad;
@@ -530,8 +583,8 @@ static int launch_gemm2(edhmc_ctx* ctx, hipStream_t st) {
 static int grad_eval(edhmc_ctx* ctx, bool split, bool ll, hipStream_t st) {
   if (ctx->cdt == EDHMC_BF16) {
     if (split) {
-      if (ll) EDHMC_TRY((launch_gemm1<bf16_t, Cfg1, 2, true, true>(ctx, st)));
-      else EDHMC_TRY((launch_gemm1<bf16_t, Cfg1, 2, true, false>(ctx, st)));
+      if (ll) EDHMC_TRY((launch_gemm1<bf16_t, CfgP2, 2, true, true>(ctx, st)));
+      else EDHMC_TRY((launch_gemm1<bf16_t, CfgP2, 2, true, false>(ctx, st)));
     } else {
       if (ll) EDHMC_TRY((launch_gemm1<bf16_t, Cfg1, 1, true, true>(ctx, st)));
       else EDHMC_TRY((launch_gemm1<bf16_t, Cfg1, 1, true, false>(ctx, st)));
@@ -545,7 +598,7 @@ static int grad_eval(edhmc_ctx* ctx, bool split, bool ll, hipStream_t st) {
 
 // Forward only (log-likelihood partials, no residuals) at z_lp, exact form.
 int glm_forward_ll(edhmc_ctx* ctx, hipStream_t st) {
-  if (ctx->cdt == EDHMC_BF16) return launch_gemm1<bf16_t, Cfg1, 2, false, true>(ctx, st);
+  if (ctx->cdt == EDHMC_BF16) return launch_gemm1<bf16_t, CfgP2, 2, false, true>(ctx, st);
   return launch_gemm1<float, Cfg1, 1, false, true>(ctx, st);
 }
 
@@ -589,6 +642,7 @@ static int ensure_chains(edhmc_ctx* ctx, int C) {
   EDHMC_TRY(dev_alloc(ctx, ctx->Gpart, (size_t)ks * C_pad * ctx->D_pad * 4, false));
   ctx->n_tiles1 = (int)(ctx->N_pad / Cfg1::BN);
   EDHMC_TRY(dev_alloc(ctx, ctx->llpart, (size_t)C_pad * ctx->n_tiles1 * 8, false));
+  EDHMC_TRY(dev_alloc(ctx, ctx->ll, (size_t)C_pad * 8));
   return EDHMC_OK;
 }
 
@@ -632,13 +686,19 @@ static int do_transition(edhmc_ctx* ctx, float eps, int L, const float* r0, cons
   a.u_fed = u;
   a.seed = seed;
   a.counter = counter;
-  hipLaunchKernelGGL(chain_begin_kernel, dim3(ctx->C), dim3(kChainThreads), 0, st, a);
+  {
+    ProfScope ps(ctx, PK_CHAIN, st);
+    hipLaunchKernelGGL(chain_begin_kernel, dim3(ctx->C), dim3(kChainThreads), 0, st, a);
+  }
   EDHMC_TRY(check_launch(ctx, "chain_begin"));
   for (int k = 1; k <= L; ++k) {
     const bool last = k == L;
     EDHMC_TRY(grad_eval(ctx, last, last, st));
     if (!last) {
-      hipLaunchKernelGGL(chain_mid_kernel, dim3(ctx->C), dim3(kChainThreads), 0, st, a);
+      {
+        ProfScope ps(ctx, PK_CHAIN, st);
+        hipLaunchKernelGGL(chain_mid_kernel, dim3(ctx->C), dim3(kChainThreads), 0, st, a);
+      }
       EDHMC_TRY(check_launch(ctx, "chain_mid"));
     }
   }
@@ -651,7 +711,10 @@ static int do_transition(edhmc_ctx* ctx, float eps, int L, const float* r0, cons
     a.mom_m2 = (double*)ctx->mom_m2.p;
     a.mom_n = ctx->mom_count;
   }
-  hipLaunchKernelGGL(chain_last_kernel, dim3(ctx->C), dim3(kChainThreads), 0, st, a);
+  {
+    ProfScope ps(ctx, PK_CHAIN, st);
+    hipLaunchKernelGGL(chain_last_kernel, dim3(ctx->C), dim3(kChainThreads), 0, st, a);
+  }
   return check_launch(ctx, "chain_last");
 }
 
@@ -660,10 +723,11 @@ static int do_transition(edhmc_ctx* ctx, float eps, int L, const float* r0, cons
 using namespace edhmc;
 
 edhmc_ctx::~edhmc_ctx() {
+  for (hipEvent_t e : prof.pool) (void)hipEventDestroy(e);
   if (sur) surrogate_destroy(sur);
   if (hier) hier_destroy(hier);
   DevBuf* bufs[] = {&Xr, &Xc, &y, &ploc, &pscale, &z, &r, &g, &z0, &g0, &z_lp, &logp, &K0,
-                    &flag, &ndiv, &Rt, &Gpart, &llpart, &mom_mean, &mom_m2, &tmp_count};
+                    &flag, &ndiv, &Rt, &Gpart, &llpart, &ll, &mom_mean, &mom_m2, &tmp_count};
   for (DevBuf* b : bufs) b->release();
 }
 
@@ -827,6 +891,29 @@ int edhmc_run(edhmc_ctx* ctx, float eps, int32_t n_steps, float* params, int64_t
     EDHMC_TRY(do_transition(ctx, eps, n_steps, nullptr, nullptr, seed, counter0 + (uint64_t)t,
                             nullptr, params + t * (int64_t)ctx->C * ctx->D, n_accept,
                             (hipStream_t)stream));
+  }
+  return EDHMC_OK;
+}
+
+int edhmc_profile_enable(edhmc_ctx* ctx, int32_t enable) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  EDHMC_TRY(prof_collect(ctx));
+  ctx->prof.on = enable != 0;
+  for (int k = 0; k < PK_COUNT; ++k) {
+    ctx->prof.total_ms[k] = 0.0;
+    ctx->prof.count[k] = 0;
+  }
+  return EDHMC_OK;
+}
+
+int edhmc_profile_read(edhmc_ctx* ctx, double* total_ms_out, int64_t* count_out) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  EDHMC_TRY(prof_collect(ctx));
+  for (int k = 0; k < PK_COUNT; ++k) {
+    if (total_ms_out) total_ms_out[k] = ctx->prof.total_ms[k];
+    if (count_out) count_out[k] = ctx->prof.count[k];
   }
   return EDHMC_OK;
 }

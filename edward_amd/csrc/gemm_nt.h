@@ -1,19 +1,26 @@
 // MFMA "NT" GEMM for gfx950:  C[m, n] = sum_k A[m, k] * B[n, k]
 //
-// Both operands are row-major with K contiguous, so every lane feeds the
-// matrix core from one 16-byte LDS read per operand and K-chunk:
+// Both operands hold K contiguously in 64-byte stages, so every lane feeds
+// the matrix core from one 16-byte LDS read per operand and K-chunk:
 //   * bf16: v_mfma_f32_32x32x16_bf16, the 16 bytes are 8 consecutive k;
 //   * fp32: v_mfma_f32_32x32x2_f32 issued 4 times, element s of the 16 bytes
 //     feeds issue s.  Lanes 0-31 and 32-63 read the two 16-byte halves of a
 //     32-byte K-chunk, so A and B see the same k <-> (lane, element)
 //     assignment and the contraction covers each k exactly once.
 //
+// Operand addressing: byte address of (row r, stage s, chunk c) is
+//   base + r * ld + s * stage + 16 c.
+// Row-major operands with a short K (positions, X rows) use ld = K bytes,
+// stage = 64.  Operands with a long K (the residuals R^T and X^T, K = data
+// rows) are stored K-blocked: ld = 64, stage = rows * 64, so the tile a stage
+// reads is one contiguous block (a row-major [1e3, 1e6] layout would put
+// every row of a tile on its own 2 MB page and thrash the TLB).
+//
 // Work decomposition: a workgroup of WAVES_M x WAVES_N wave64s owns a
 // BM x BN output tile (each wave WM x WN blocks of 32 x 32) and a range of
-// K stages (split-K).  Stages of 64 bytes per row are double-buffered in LDS
-// (row stride 80 bytes: the 8/16 lanes served per LDS cycle by a
-// ds_read_b128 hit disjoint banks).  The next stage's global loads are in
-// flight while the current stage feeds the MFMAs.
+// K stages (split-K).  Stages are double-buffered in LDS (row stride 80
+// bytes: the lanes served per LDS cycle by a ds_read_b128 hit disjoint
+// banks), and the global loads run two stages ahead in registers.
 //
 // Tile order is XCD-aware: consecutive workgroup ids go round-robin to the
 // 8 XCDs, so ids are mapped such that all workgroups that read the same B
@@ -30,9 +37,11 @@ constexpr int kRowStride = 80;
 struct GemmShape {
   const char* A;      // [planes][M][K]
   int64_t lda_b;      // bytes between rows of A
+  int64_t a_stage_b;  // bytes between K stages of A
   int64_t a_plane_b;  // bytes between planes of A
   const char* B;      // [N][K]
   int64_t ldb_b;
+  int64_t b_stage_b;
   int m_tiles, n_tiles;
   int k_stages;          // K bytes / 64
   int ksplit;            // number of K ranges
@@ -116,7 +125,7 @@ struct EpiPos {
 };
 
 template <typename T, typename Cfg, int A_PLANES, typename Epi>
-__global__ void __launch_bounds__(Cfg::kThreads)
+__global__ void __launch_bounds__(Cfg::kThreads) __attribute__((amdgpu_waves_per_eu(1, 2)))
 gemm_nt_kernel(GemmShape g, Epi epi) {
   constexpr int NT = Cfg::kThreads;
   constexpr int WM = Cfg::kWM, WN = Cfg::kWN, WAVES_M = Cfg::kWavesM;
@@ -137,6 +146,7 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
   const int s_end = min(g.k_stages, s_begin + g.stages_per_split);
 
   const char* src[PER_T];
+  int64_t sstride[PER_T];
   int dst[PER_T];
 #pragma unroll
   for (int q = 0; q < PER_T; ++q) {
@@ -146,9 +156,11 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
     if (row < AR) {
       int p = row / BM, r = row - p * BM;
       src[q] = g.A + p * g.a_plane_b + (int64_t)(tm * BM + r) * g.lda_b + col * 16;
+      sstride[q] = g.a_stage_b;
     } else {
       int r = row - AR;
       src[q] = g.B + (int64_t)(tn * BN + r) * g.ldb_b + col * 16;
+      sstride[q] = g.b_stage_b;
     }
     dst[q] = row * kRowStride + col * 16;
   }
@@ -161,25 +173,8 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
-  uint4 pre[PER_T];
-  if (s_begin < s_end) {
-#pragma unroll
-    for (int q = 0; q < PER_T; ++q)
-      pre[q] = *(const uint4*)(src[q] + (int64_t)s_begin * kStageBytes);
-#pragma unroll
-    for (int q = 0; q < PER_T; ++q) *(uint4*)(smem + dst[q]) = pre[q];
-  }
-  __syncthreads();
-
   const int lrow = lane & 31, lhalf = (lane >> 5) * 16;
-  for (int s = s_begin; s < s_end; ++s) {
-    const int buf = (s - s_begin) & 1;
-    const bool more = s + 1 < s_end;
-    if (more) {
-#pragma unroll
-      for (int q = 0; q < PER_T; ++q)
-        pre[q] = *(const uint4*)(src[q] + (int64_t)(s + 1) * kStageBytes);
-    }
+  auto compute = [&](int buf) {
     const char* base = smem + buf * BUF;
 #pragma unroll
     for (int kc = 0; kc < kStageBytes / 32; ++kc) {
@@ -201,13 +196,41 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
 #pragma unroll
           for (int p = 0; p < A_PLANES; ++p) Mfma<T>::chunk(acc[i][j], a[p][i], b[j]);
     }
-    if (more) {
-      char* nb = smem + (buf ^ 1) * BUF;
-#pragma unroll
-      for (int q = 0; q < PER_T; ++q) *(uint4*)(nb + dst[q]) = pre[q];
-    }
-    __syncthreads();
+  };
+
+  // Two register sets: CUR holds stage s+1 (stored to LDS this iteration),
+  // NXT receives stage s+2.  Written out (no arrays passed by reference) so
+  // the sets stay in VGPRs.
+  uint4 r0[PER_T], r1[PER_T];
+#define EDHMC_GLOAD(PRE, S)                                              \
+  _Pragma("unroll") for (int q = 0; q < PER_T; ++q)                      \
+      PRE[q] = *(const uint4*)(src[q] + (int64_t)(S) * sstride[q]);
+#define EDHMC_SSTORE(PRE, BUFI)                                          \
+  _Pragma("unroll") for (int q = 0; q < PER_T; ++q)                      \
+      *(uint4*)(smem + (BUFI) * BUF + dst[q]) = PRE[q];
+#define EDHMC_STEP(S, CUR, NXT)                                          \
+  {                                                                      \
+    const int buf_ = ((S) - s_begin) & 1;                                \
+    if ((S) + 2 < s_end) { EDHMC_GLOAD(NXT, (S) + 2) }                   \
+    compute(buf_);                                                       \
+    if ((S) + 1 < s_end) { EDHMC_SSTORE(CUR, buf_ ^ 1) }                 \
+    __syncthreads();                                                     \
   }
+  if (s_begin < s_end) {
+    EDHMC_GLOAD(r0, s_begin)
+    EDHMC_SSTORE(r0, 0)
+    if (s_begin + 1 < s_end) { EDHMC_GLOAD(r1, s_begin + 1) }
+  }
+  __syncthreads();
+  int s = s_begin;
+  for (; s + 1 < s_end; s += 2) {
+    EDHMC_STEP(s, r1, r0)
+    EDHMC_STEP(s + 1, r0, r1)
+  }
+  if (s < s_end) EDHMC_STEP(s, r1, r0)
+#undef EDHMC_STEP
+#undef EDHMC_SSTORE
+#undef EDHMC_GLOAD
 
   EpiPos pos;
   pos.m0 = tm * BM + wm * 32 * WM;

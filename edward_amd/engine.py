@@ -49,6 +49,19 @@ class Context(object):
   def check(self, rc):
     L.check(rc, self.h)
 
+  PROFILE_KINDS = ('gemm_grad', 'gemm_ll', 'gemm_xtr', 'chain', 'mlp', 'hier')
+
+  def profile(self, on=True):
+    """Turn the library's per-launch hipEvent timing on (clears totals) or off."""
+    self.check(self.lib.edhmc_profile_enable(self.h, 1 if on else 0))
+
+  def profile_read(self):
+    """{kind: (total_ms, launches)} summed over the launches since enabling."""
+    ms = (ctypes.c_double * 6)()
+    n = (ctypes.c_int64 * 6)()
+    self.check(self.lib.edhmc_profile_read(self.h, ms, n))
+    return {k: (ms[i], int(n[i])) for i, k in enumerate(self.PROFILE_KINDS)}
+
   @property
   def stream(self):
     return L.stream_ptr(torch.cuda.current_stream(self.device))

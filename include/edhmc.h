@@ -127,6 +127,19 @@ int edhmc_moments_enable(edhmc_ctx* ctx, int32_t enable);
 int edhmc_moments_get(edhmc_ctx* ctx, double* mean_out, double* m2_out,
                       int64_t* count_out, void* stream);
 
+/* Event profiler: with enable != 0, every kernel launch is bracketed by
+ * hipEvents recorded on its stream; edhmc_profile_read synchronises and
+ * returns, per kernel class, the summed duration (ms) and launch count:
+ *   0 GEMM1 gradient pass (X w with residual epilogue)
+ *   1 GEMM1 forward with FP64 log-likelihood epilogue only
+ *   2 GEMM2 (X^T residual, split-K)
+ *   3 per-chain leapfrog / accept kernels
+ *   4 surrogate MLP GEMMs
+ *   5 hierarchical persistent kernel
+ * total_ms_out, count_out: host arrays of 6.  Enabling clears the totals. */
+int edhmc_profile_enable(edhmc_ctx* ctx, int32_t enable);
+int edhmc_profile_read(edhmc_ctx* ctx, double* total_ms_out, int64_t* count_out);
+
 /* Number of chains whose proposal was non-finite (rejected) since binding,
  * [C] int32 device array copy. */
 int edhmc_divergences(edhmc_ctx* ctx, int32_t* out, void* stream);
