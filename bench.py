@@ -1,0 +1,295 @@
+"""Benchmark: min-ESS/sec of Bayesian logistic-regression HMC on one MI355X,
+surrogate-driven bf16 HMC versus plain fp32 HMC.
+
+Workload (default): N = 1e6 rows, D = 512 features (X ~ N(0,1) with column
+scales U(0.5, 2), stored bf16), 4096 chains from N(0, I), eps = 5e-4,
+L = 32 leapfrog steps -- the large logistic-regression configuration.
+
+Flow (one JSON line on stdout at the end; progress on stderr):
+  1. warm-up (untimed, W transitions): plain bf16 HMC from N(0, I); the last
+     two transitions record (w, exact gradient) at every inner leapfrog
+     point (~254k pairs) -- the surrogate's training set;
+  2. surrogate training on the GPU (untimed; reported);
+  3. timed: K surrogate transitions (exact bf16/FP64 log joint in every
+     Metropolis test), samples kept on the device; min-ESS over the D
+     coordinates from the device autocovariances -> `value`;
+  4. timed: Kp plain fp32 transitions from the same equilibrium states ->
+     the comparator's min-ESS/sec and the speedup;
+  5. the CPU restatement timed on a bounded sample on the host cores.
+`roofline` is the dominant kernel of the timed surrogate transitions,
+measured with hipEvents recorded by the library on its launch stream.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+  sys.path.insert(0, ROOT)
+
+PEAK = {'bf16': 2.5e15, 'fp32': 157.3e12, 'hbm': 8.0e12}
+
+
+def log(msg):
+  sys.stderr.write('[bench %7.1fs] %s\n' % (time.time() - T0, msg))
+  sys.stderr.flush()
+
+
+T0 = time.time()
+
+
+def parse():
+  ap = argparse.ArgumentParser()
+  ap.add_argument('--gpus', type=int, default=1)
+  ap.add_argument('--steps', type=int, default=200, help='timed surrogate transitions')
+  ap.add_argument('--warmup', type=int, default=40, help='max plain bf16 warm-up transitions')
+  ap.add_argument('--plain-steps', type=int, default=8, help='timed plain fp32 transitions')
+  ap.add_argument('--workload', default='large', choices=['large', 'mid', 'small'])
+  ap.add_argument('--hidden', type=int, default=1024)
+  ap.add_argument('--epochs', type=int, default=12)
+  ap.add_argument('--seed', type=int, default=0)
+  ap.add_argument('--no-cpu', action='store_true')
+  ap.add_argument('--cpu-seconds', type=float, default=15.0)
+  return ap.parse_args()
+
+
+def make_workload(name):
+  from edward_amd import workloads as W
+  if name == 'large':
+    return W.logreg_large()
+  if name == 'mid':
+    wl = W.logreg(100000, 256, 1024, 0.002, 20, 2000, bf16=True)
+    return wl
+  wl = W.logreg(20000, 64, 512, 0.005, 16, 1000, bf16=True)
+  return wl
+
+
+def cpu_baseline(wl, budget_s, ess_per_transition_plain):
+  """The oracle's fp32 restatement of one plain transition's work: L
+  gradient evaluations (eta = W X^T, R = y - sigmoid(eta), G = R X), timed
+  on a sample of chains at the full N on the host cores (OpenBLAS)."""
+  from oracle import hmc_oracle as O  # the CPU restatement (checker) only
+  cores = len(os.sched_getaffinity(0))
+  X = wl.X
+  y = wl.y
+  Cs = 16
+  W = wl.W0[:Cs].astype(np.float32)
+  reps, t = 0, 0.0
+  while t < budget_s and reps < 50:
+    t0 = time.perf_counter()
+    eta = W @ X.T
+    R = (y[None, :] - O.sigmoid(eta)).astype(np.float32)
+    G = R @ X
+    t += time.perf_counter() - t0
+    reps += 1
+  sec_per_eval = t / reps * (wl.C / Cs)
+  sec_per_transition = sec_per_eval * wl.n_steps
+  return {
+      'value': ess_per_transition_plain / sec_per_transition,
+      'unit': 'min-ESS/s (derived: GPU fp32 ESS per transition / CPU sec per transition)',
+      'cores': cores,
+      'kind': 'port',
+      'sample': ('%d gradient evaluations of %d of %d chains at N=%d, D=%d in fp32 numpy '
+                 '(OpenBLAS, %d threads); scaled x%d chains x L=%d -> %.1f s per transition'
+                 % (reps, Cs, wl.C, wl.N, wl.D, cores, wl.C // Cs, wl.n_steps,
+                    sec_per_transition)),
+      'sec_per_transition': sec_per_transition,
+  }
+
+
+def main():
+  args = parse()
+  if args.gpus != 1:
+    print('bench.py measures one GPU: this build runs one process on one MI355X; '
+          'multi-GPU scaling is not part of it (requested --gpus %d)' % args.gpus,
+          file=sys.stderr)
+    sys.exit(2)
+  import torch
+  from edward_amd import diagnostics
+  from edward_amd.engine import GLMEngine
+  from edward_amd.surrogate import Surrogate
+
+  wl = make_workload(args.workload)
+  N, D, C, eps, L = wl.N, wl.D, wl.C, wl.step_size, wl.n_steps
+  log('workload N=%d D=%d C=%d eps=%g L=%d' % (N, D, C, eps, L))
+  dev = torch.device('cuda', 0)
+  bf = GLMEngine(wl.X, wl.y, wl.prior_loc, wl.prior_scale, precision='bf16')
+  bf.set_state(wl.W0)
+  torch.cuda.synchronize()
+
+  # 1. warm-up with plain bf16 HMC; record pairs over the last two transitions
+  t_warm0 = time.time()
+  means = []
+  n_rec = 2
+  cap = (L - 1) * C * n_rec
+  Wrec = torch.empty(cap, D, device=dev)
+  Grec = torch.empty(cap, D, device=dev)
+  n_acc = torch.zeros(C, dtype=torch.int32, device=dev)
+  t = 0
+  converged_at = None
+  jit = np.random.default_rng(args.seed + 7)
+  while t < args.warmup:
+    # the step size is jittered log-uniformly over [eps/2, 2 eps] during
+    # warm-up: with a fixed eps L, posterior directions whose frequency makes
+    # eps L a multiple of half a period keep their energy through every
+    # momentum refresh and approach the typical set very slowly
+    eps_t = eps * 2.0 ** jit.uniform(-1.0, 1.0)
+    n_acc.zero_()
+    bf.transition(eps_t, L, seed=args.seed, counter=t, n_accept=n_acc)
+    _, lp, _ = bf.get_state()
+    means.append(lp.mean().item())
+    log('warm-up %d eps %.2e mean logp %.2f accept %.3f'
+        % (t, eps_t, means[-1], n_acc.float().mean().item()))
+    t += 1
+    if converged_at is None and t >= 10 and abs(means[-1] - means[-6]) < 20.0:
+      converged_at = t
+      break
+  bf.record_pairs(Wrec, Grec)
+  for k in range(n_rec):
+    bf.transition(eps, L, seed=args.seed, counter=t, n_accept=n_acc)
+    t += 1
+  bf.record_pairs(None)
+  torch.cuda.synchronize()
+  n_pairs = bf.record_count()
+  warm_s = time.time() - t_warm0
+  log('warm-up done: %d transitions, %d pairs, %.1f s' % (t, n_pairs, warm_s))
+
+  # 2. train the surrogate
+  t_tr0 = time.time()
+  sur = Surrogate(bf, hidden=args.hidden, max_batch=C, seed=args.seed)
+  losses = sur.fit(Wrec[:n_pairs], Grec[:n_pairs], epochs=args.epochs, batch=C, lr=1e-3,
+                   lr_final=5e-5, seed=args.seed, log=log)
+  torch.cuda.synchronize()
+  train_s = time.time() - t_tr0
+  del Wrec, Grec
+  log('training done: %.1f s, loss %.4f -> %.4f' % (train_s, losses[0], losses[-1]))
+
+  # 3. timed surrogate sampling
+  K = args.steps
+  samples = torch.empty(K, C, D, device=dev)
+  n_acc.zero_()
+  for k in range(2):   # untimed: settle ghat and first-call overheads
+    sur.transition(eps, L, seed=args.seed, counter=t, n_accept=None)
+    t += 1
+  bf.profile(True)
+  torch.cuda.synchronize()
+  ts0 = time.time()
+  for k in range(K):
+    sur.transition(eps, L, seed=args.seed, counter=t, sample_out=samples[k], n_accept=n_acc)
+    t += 1
+  torch.cuda.synchronize()
+  sur_s = time.time() - ts0
+  prof = bf.profile_read()
+  bf.profile(False)
+  acc_sur = n_acc.float().mean().item() / K
+  ess_sur = diagnostics.ess(bf, samples)
+  mess_sur = float(ess_sur.min())
+  mess_sur_within = float(diagnostics.ess(bf, samples, between=False).min())
+  Kp = args.plain_steps
+  mess_sur_kp = float(diagnostics.ess(bf, samples[:Kp]).min())
+  log('surrogate: %d transitions %.2f s, accept %.3f, min-ESS %.1f (median %.1f)'
+      % (K, sur_s, acc_sur, mess_sur, float(np.median(ess_sur))))
+  state, _, _ = bf.get_state()
+  sur_mean = samples.double().mean((0, 1)).cpu().numpy()
+  sur_var = samples.double().var((0, 1)).cpu().numpy()
+  del samples
+  torch.cuda.empty_cache()
+
+  # 4. timed plain fp32 from the same equilibrium states
+  f32 = GLMEngine(wl.X, wl.y, wl.prior_loc, wl.prior_scale, precision='fp32')
+  f32.set_state(state)
+  psamples = torch.empty(Kp, C, D, device=dev)
+  n_acc.zero_()
+  torch.cuda.synchronize()
+  tp0 = time.time()
+  for k in range(Kp):
+    f32.transition(eps, L, seed=args.seed + 1, counter=t + k, sample_out=psamples[k],
+                   n_accept=n_acc)
+    torch.cuda.synchronize()
+    log('plain fp32 transition %d' % k)
+  plain_s = time.time() - tp0
+  acc_plain = n_acc.float().mean().item() / Kp
+  ess_plain = diagnostics.ess(f32, psamples)
+  mess_plain = float(ess_plain.min())
+  mess_plain_within = float(diagnostics.ess(f32, psamples, between=False).min())
+  log('plain fp32: %d transitions %.2f s, accept %.3f, min-ESS %.1f'
+      % (Kp, plain_s, acc_plain, mess_plain))
+  plain_mean = psamples.double().mean((0, 1)).cpu().numpy()
+  del psamples, f32
+  torch.cuda.empty_cache()
+
+  sur_rate = mess_sur / sur_s
+  plain_rate = mess_plain / plain_s
+  # equal window: the surrogate's first Kp transitions against the plain Kp
+  sur_rate_kp = mess_sur_kp / (sur_s * Kp / K)
+  incl_rate = mess_sur / (sur_s + warm_s + train_s)
+
+  # roofline of the dominant kernel of the timed surrogate transitions
+  kinds = {k: v for k, v in prof.items() if v[1] > 0}
+  dom = max(kinds, key=lambda k: kinds[k][0])
+  ms, cnt = kinds[dom]
+  if dom == 'gemm_ll':
+    flop = 4.0 * N * D * C          # two bf16 products (hi/lo split positions)
+  elif dom == 'mlp':
+    H = args.hidden
+    flop = 2.0 * C * (D * H + H * H + H * D) / 3.0   # mean over the 3 layer GEMMs
+  else:
+    flop = 2.0 * N * D * C
+  achieved = flop / (ms / cnt * 1e-3)
+  roof = {'bound': 'mfma', 'kernel': dom, 'achieved': achieved / 1e12,
+          'peak': PEAK['bf16'] / 1e12, 'unit': 'TFLOP/s', 'frac': achieved / PEAK['bf16'],
+          'traffic': None, 'avg_launch_ms': ms / cnt,
+          'per_transition_ms': {k: v[0] / K for k, v in kinds.items()}}
+
+  cpu = None
+  if not args.no_cpu:
+    cpu = cpu_baseline(wl, args.cpu_seconds, mess_plain / Kp)
+    log('cpu baseline: %s' % cpu['sample'])
+
+  out = {
+      'metric': 'min-ESS/sec, logreg HMC N=1M D=512 4096 chains: surrogate+BF16 vs plain FP32',
+      'value': sur_rate,
+      'unit': 'min-ESS/s',
+      'n_gpus': 1,
+      'steps': K,
+      'warmup': t - K - 2,
+      'ms_per_step': sur_s / K * 1e3,
+      'higher_is_better': True,
+      'scaling': 'weak',
+      'vs_baseline': None,
+      'dtype': 'bf16',
+      'data': 'synthetic',
+      'config': {'workload': 'logreg_hmc N=%d D=%d chains=%d eps=%g L=%d X=bf16 '
+                             'surrogate MLP %d-%d-%d-%d' % (N, D, C, eps, L, D, args.hidden,
+                                                            args.hidden, D),
+                 'global_batch': C, 'seq_len': None, 'parallelism': 'none (1 GPU)'},
+      'ess_estimator': 'multi-chain (Gelman et al. BDA3 11.5, Geyer initial monotone '
+                       'sequence) over the timed transitions, min over the D coordinates',
+      'plain_fp32': {'min_ess_per_s': plain_rate, 'min_ess': mess_plain, 'transitions': Kp,
+                     'seconds': plain_s, 'accept': acc_plain,
+                     'min_ess_within_chain': mess_plain_within},
+      'surrogate_bf16': {'min_ess_per_s': sur_rate, 'min_ess': mess_sur, 'transitions': K,
+                         'seconds': sur_s, 'accept': acc_sur,
+                         'min_ess_per_s_incl_warmup_and_training': incl_rate,
+                         'warmup_s': warm_s, 'train_s': train_s, 'train_pairs': n_pairs,
+                         'final_loss': losses[-1], 'min_ess_within_chain': mess_sur_within,
+                         'min_ess_first_%d' % Kp: mess_sur_kp},
+      'speedup_vs_plain_fp32': sur_rate / plain_rate if plain_rate > 0 else None,
+      'speedup_equal_window': sur_rate_kp / plain_rate if plain_rate > 0 else None,
+      'speedup_incl_training': incl_rate / plain_rate if plain_rate > 0 else None,
+      'posterior_mean_max_abs_diff_sur_vs_plain': float(np.abs(sur_mean - plain_mean).max()),
+      'posterior_sd_median': float(np.median(np.sqrt(sur_var))),
+      'roofline': roof,
+      'cpu_baseline': None if cpu is None else {k: cpu[k] for k in
+                                                ('value', 'unit', 'cores', 'kind', 'sample')},
+  }
+  print(json.dumps(out))
+
+
+if __name__ == '__main__':
+  main()

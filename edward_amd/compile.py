@@ -165,8 +165,26 @@ def _normal_prior(rv, k):
   return loc, scale
 
 
+class GLMSpec(object):
+  """What the GLM engine is bound with: design matrix, observations, prior,
+  likelihood, and the latent layout."""
+
+  def __init__(self, **kw):
+    self.__dict__.update(kw)
+
+
 def compile_model(latent_vars, data, precision='fp32'):
+  spec = glm_spec(latent_vars, data)
+  if not isinstance(spec, GLMSpec):
+    return spec          # another engine family built its own program
   from edward_amd.engine import GLMEngine
+  eng = GLMEngine(spec.X, spec.y, spec.loc, spec.scale, likelihood=spec.likelihood,
+                  lik_scale=spec.lik_scale, precision=precision)
+  return Program(eng, spec.latents, spec.n_rows, spec.n_chains, spec.dim)
+
+
+def glm_spec(latent_vars, data):
+  """Match the model to the GLM family; returns a GLMSpec (no device work)."""
   observed = [k for k in data if isinstance(k, M.RandomVariable)]
   if len(observed) != 1:
     raise NotImplementedError('exactly one observed random variable is supported, got %d'
@@ -222,9 +240,9 @@ def compile_model(latent_vars, data, precision='fp32'):
     off += k
   X = np.concatenate(blocks, axis=1).astype(np.float32)
   T = int(min(l.qz.params.shape[0] for l in lat))
-  eng = GLMEngine(X, yv.astype(np.float32), np.concatenate(locs), np.concatenate(scales),
-                  likelihood=likelihood, lik_scale=lik_scale, precision=precision)
-  return Program(eng, lat, T, n_chains, off)
+  return GLMSpec(X=X, y=yv.astype(np.float32), loc=np.concatenate(locs),
+                 scale=np.concatenate(scales), likelihood=likelihood, lik_scale=lik_scale,
+                 latents=lat, n_rows=T, n_chains=n_chains, dim=off)
 
 
 def _has_gather(E):

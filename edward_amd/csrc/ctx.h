@@ -76,6 +76,7 @@ struct edhmc_ctx {
   // ---- chains ----
   int C = 0, C_pad = 0;
   bool state_valid = false;
+  bool g_valid = false;   // g holds the exact gradient at z
   edhmc::DevBuf z, r, g, z0, g0, z_lp, logp, K0, flag, ndiv;
   // ---- gradient scratch ----
   edhmc::DevBuf Rt, Gpart, llpart, ll;
@@ -85,8 +86,12 @@ struct edhmc_ctx {
   bool mom_on = false;
   int64_t mom_count = 0;
   edhmc::DevBuf mom_mean, mom_m2;
+  // ---- training-pair recording ----
+  float* rec_W = nullptr;
+  float* rec_G = nullptr;
+  int64_t rec_cap = 0, rec_count = 0;
   // ---- misc scratch ----
-  edhmc::DevBuf tmp_count;
+  edhmc::DevBuf tmp_count, stats_part;
 
   edhmc::Profiler prof;
   edhmc::Surrogate* sur = nullptr;
@@ -168,6 +173,7 @@ struct DeviceGuard {
 
 // Implemented in the other translation units.
 void surrogate_destroy(Surrogate* s);
+void surrogate_invalidate(Surrogate* s);   // chain state changed under it
 void hier_destroy(HierModel* h);
 
 // GLM pieces reused by the surrogate transition (edhmc_glm.hip).

@@ -209,6 +209,7 @@ struct ChainArgs {
   double* mom_mean; double* mom_m2; int64_t mom_n;   // mom_n = count after update
   float* grad_out; double* logp_out;
   const float* W_in;        // set_state input [C, D]
+  float* rec_W; float* rec_G;   // training-pair rows for this step (or null)
 };
 
 __device__ inline void write_lp(const ChainArgs& a, int c, int d, float zv) {
@@ -291,6 +292,10 @@ __global__ void __launch_bounds__(kChainThreads) chain_mid_kernel(ChainArgs a) {
     const float zv = a.z[idx];
     const float gv = gsum(a, c, d) + prior_grad(a, d, zv);
     a.g[idx] = gv;
+    if (a.rec_W) {
+      a.rec_W[(int64_t)c * a.D + d] = zv;
+      a.rec_G[(int64_t)c * a.D + d] = gv;
+    }
     float r = a.r[idx];
     r = r + a.half * gv;
     r = r + a.half * gv;
@@ -478,6 +483,62 @@ autocov_kernel(const float* __restrict__ s, int64_t T, int C, int D, int max_lag
     for (int k = 0; k < 16; ++k)
       if (l0 + k <= max_lag) atomicAdd(&acov[(int64_t)(l0 + k) * D + d], acc[k] / (double)T);
   }
+}
+
+// Column statistics, pass 1: per (row chunk, column) FP64 sum and sum of
+// squares; pass 2: fixed-order combination.
+__global__ void __launch_bounds__(256)
+col_stats_part_kernel(const float* __restrict__ X, int64_t P, int D, int chunks,
+                      double* __restrict__ part) {
+  const int d = blockIdx.x * 256 + threadIdx.x;
+  const int ch = blockIdx.y;
+  if (d >= D) return;
+  const int64_t per = (P + chunks - 1) / chunks;
+  const int64_t b = ch * per, e = min(P, b + per);
+  double s = 0.0, q = 0.0;
+  for (int64_t i = b; i < e; ++i) {
+    const double v = X[i * D + d];
+    s += v;
+    q += v * v;
+  }
+  part[((int64_t)ch * D + d) * 2] = s;
+  part[((int64_t)ch * D + d) * 2 + 1] = q;
+}
+
+__global__ void col_stats_final_kernel(const double* __restrict__ part, int64_t P, int D,
+                                       int chunks, float* __restrict__ mean,
+                                       float* __restrict__ sd) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= D) return;
+  double s = 0.0, q = 0.0;
+  for (int ch = 0; ch < chunks; ++ch) {
+    s += part[((int64_t)ch * D + d) * 2];
+    q += part[((int64_t)ch * D + d) * 2 + 1];
+  }
+  const double m = s / (double)P;
+  const double v = fmax(q / (double)P - m * m, 0.0);
+  mean[d] = (float)m;
+  sd[d] = (float)sqrt(v);
+}
+
+// Empirical moments: thread per column, two passes over rows t0..T-1.
+template <typename P>
+__global__ void __launch_bounds__(256)
+sample_moments_kernel(const P* __restrict__ x, int64_t T, int64_t K, int64_t t0,
+                      double* __restrict__ mean, double* __restrict__ var) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  double s = 0.0;
+  for (int64_t t = t0; t < T; ++t) s += (double)x[t * K + k];
+  const double n = (double)(T - t0);
+  const double m = s / n;
+  double q = 0.0;
+  for (int64_t t = t0; t < T; ++t) {
+    const double d = (double)x[t * K + k] - m;
+    q += d * d;
+  }
+  mean[k] = m;
+  var[k] = q / n;
 }
 
 // ---------------------------------------------------------------------------
@@ -669,6 +730,8 @@ static int do_set_state(edhmc_ctx* ctx, const float* W, int C, hipStream_t st) {
   hipLaunchKernelGGL(chain_eval_kernel, dim3(C), dim3(kChainThreads), 0, st, a);
   EDHMC_TRY(check_launch(ctx, "chain_eval"));
   ctx->state_valid = true;
+  ctx->g_valid = true;
+  surrogate_invalidate(ctx->sur);
   return EDHMC_OK;
 }
 
@@ -679,6 +742,16 @@ static int do_transition(edhmc_ctx* ctx, float eps, int L, const float* r0, cons
     return set_err(ctx, EDHMC_ESTATE, "no chain state: call edhmc_set_state first");
   if (L < 1) return set_err(ctx, EDHMC_EINVAL, "n_steps must be >= 1, got %d", L);
   if (!(std::isfinite(eps))) return set_err(ctx, EDHMC_EINVAL, "step_size must be finite");
+  if (!ctx->g_valid) {
+    // the previous transitions were surrogate-driven: exact gradient at z
+    EDHMC_TRY(glm_write_lp(ctx, st));
+    EDHMC_TRY(grad_eval(ctx, true, true, st));
+    ChainArgs e = chain_args(ctx);
+    hipLaunchKernelGGL(chain_eval_kernel, dim3(ctx->C), dim3(kChainThreads), 0, st, e);
+    EDHMC_TRY(check_launch(ctx, "chain_eval"));
+    ctx->g_valid = true;
+  }
+  surrogate_invalidate(ctx->sur);
   ChainArgs a = chain_args(ctx);
   a.half = (float)(0.5 * (double)eps);
   a.full = eps;
@@ -695,6 +768,12 @@ static int do_transition(edhmc_ctx* ctx, float eps, int L, const float* r0, cons
     const bool last = k == L;
     EDHMC_TRY(grad_eval(ctx, last, last, st));
     if (!last) {
+      a.rec_W = a.rec_G = nullptr;
+      if (ctx->rec_W && ctx->rec_count + ctx->C <= ctx->rec_cap) {
+        a.rec_W = ctx->rec_W + ctx->rec_count * ctx->D;
+        a.rec_G = ctx->rec_G + ctx->rec_count * ctx->D;
+        ctx->rec_count += ctx->C;
+      }
       {
         ProfScope ps(ctx, PK_CHAIN, st);
         hipLaunchKernelGGL(chain_mid_kernel, dim3(ctx->C), dim3(kChainThreads), 0, st, a);
@@ -727,7 +806,8 @@ edhmc_ctx::~edhmc_ctx() {
   if (sur) surrogate_destroy(sur);
   if (hier) hier_destroy(hier);
   DevBuf* bufs[] = {&Xr, &Xc, &y, &ploc, &pscale, &z, &r, &g, &z0, &g0, &z_lp, &logp, &K0,
-                    &flag, &ndiv, &Rt, &Gpart, &llpart, &ll, &mom_mean, &mom_m2, &tmp_count};
+                    &flag, &ndiv, &Rt, &Gpart, &llpart, &ll, &mom_mean, &mom_m2, &tmp_count,
+                    &stats_part};
   for (DevBuf* b : bufs) b->release();
 }
 
@@ -986,6 +1066,56 @@ int edhmc_draw_uniforms(edhmc_ctx* ctx, int32_t C, uint64_t seed, uint64_t count
   hipLaunchKernelGGL(rng_uniforms_kernel, dim3((C + 255) / 256), dim3(256), 0, (hipStream_t)stream,
                      C, seed, counter, u_out);
   return check_launch(ctx, "rng_uniforms");
+}
+
+int edhmc_sample_moments(edhmc_ctx* ctx, const void* params, int32_t params_dtype, int64_t T,
+                         int64_t K, int64_t t0, double* mean_out, double* var_out, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  if (!params || !mean_out || !var_out || T <= 0 || K <= 0 || t0 < 0 || t0 >= T)
+    return set_err(ctx, EDHMC_EINVAL, "bad arguments (T=%lld K=%lld t0=%lld)", (long long)T,
+                   (long long)K, (long long)t0);
+  if (params_dtype != EDHMC_F32 && params_dtype != 2)
+    return set_err(ctx, EDHMC_EINVAL, "params dtype must be fp32 or fp64");
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned blocks = (unsigned)((K + 255) / 256);
+  if (params_dtype == EDHMC_F32)
+    hipLaunchKernelGGL(sample_moments_kernel<float>, dim3(blocks), dim3(256), 0, st,
+                       (const float*)params, T, K, t0, mean_out, var_out);
+  else
+    hipLaunchKernelGGL(sample_moments_kernel<double>, dim3(blocks), dim3(256), 0, st,
+                       (const double*)params, T, K, t0, mean_out, var_out);
+  return check_launch(ctx, "sample_moments");
+}
+
+int edhmc_record_pairs(edhmc_ctx* ctx, float* W_buf, float* G_buf, int64_t capacity) {
+  if (!ctx) return EDHMC_EINVAL;
+  if (W_buf && (!G_buf || capacity <= 0))
+    return set_err(ctx, EDHMC_EINVAL, "recording needs both buffers and a positive capacity");
+  ctx->rec_W = W_buf;
+  ctx->rec_G = W_buf ? G_buf : nullptr;
+  ctx->rec_cap = W_buf ? capacity : 0;
+  if (W_buf) ctx->rec_count = 0;   // stopping keeps the count readable
+  return EDHMC_OK;
+}
+
+int64_t edhmc_record_count(edhmc_ctx* ctx) { return ctx ? ctx->rec_count : -1; }
+
+int edhmc_col_stats(edhmc_ctx* ctx, const float* X, int64_t P, int32_t D, float* mean_out,
+                    float* std_out, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  if (!X || !mean_out || !std_out || P <= 0 || D <= 0)
+    return set_err(ctx, EDHMC_EINVAL, "bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const int chunks = (int)std::min<int64_t>(256, P);
+  EDHMC_TRY(dev_alloc(ctx, ctx->stats_part, (size_t)chunks * D * 16, false));
+  hipLaunchKernelGGL(col_stats_part_kernel, dim3((D + 255) / 256, chunks), dim3(256), 0, st, X, P,
+                     D, chunks, (double*)ctx->stats_part.p);
+  EDHMC_TRY(check_launch(ctx, "col_stats"));
+  hipLaunchKernelGGL(col_stats_final_kernel, dim3((D + 255) / 256), dim3(256), 0, st,
+                     (const double*)ctx->stats_part.p, P, D, chunks, mean_out, std_out);
+  return check_launch(ctx, "col_stats_final");
 }
 
 int edhmc_autocov(edhmc_ctx* ctx, const float* s, int64_t T, int32_t C, int32_t D, int32_t max_lag,

@@ -1,14 +1,935 @@
+// Learned gradient surrogate: an MLP D -> H -> H -> D (tanh-GELU) whose output
+// layer also sees the normalised input (a linear skip: out = W3 [h2 | x] + b3),
+// trained on (w, grad log p(w | data)) pairs, then used as the leapfrog force.
+// Near a log-concave posterior's mode the gradient is close to linear in w;
+// the skip represents that part exactly and the hidden layers learn the rest.
+//
+// No reference counterpart (Edward's HMC always differentiates the log
+// joint, hmc.py:199,206).  Exactness: leapfrog with any position-only force
+// field is a composition of shears, so it is volume preserving and
+// reversible; the Metropolis test still uses the exact log joint
+// (hmc.py:100-111 with the exact _log_joint), so the chain targets the exact
+// posterior (Neal 2011, "MCMC using Hamiltonian dynamics", sec. 5.5; Li et al.
+// 2019, "Neural network gradient Hamiltonian Monte Carlo").
+//
+// Numerics: fp32 master weights, bf16 copies on the matrix cores (the same
+// MFMA NT-GEMM as the log joint) with fp32 accumulation, fp32 Adam.
+// Layouts are padded to 128 in every feature dimension and the batch; pads
+// hold zeros, so they contribute nothing (GELU(0) = 0).
+#include <algorithm>
+#include <cmath>
+
+#include "common.h"
 #include "ctx.h"
-namespace edhmc { struct Surrogate {}; void surrogate_destroy(Surrogate* s) { delete s; } }
-using namespace edhmc;
-extern "C" {
-int edhmc_sur_create(edhmc_ctx* ctx, int32_t, int32_t, int32_t) { return set_err(ctx, EDHMC_EUNSUPPORTED, "todo"); }
-int64_t edhmc_sur_num_params(edhmc_ctx*) { return 0; }
-int edhmc_sur_set_params(edhmc_ctx* ctx, const float*, void*) { return set_err(ctx, EDHMC_EUNSUPPORTED, "todo"); }
-int edhmc_sur_get_params(edhmc_ctx* ctx, float*, void*) { return set_err(ctx, EDHMC_EUNSUPPORTED, "todo"); }
-int edhmc_sur_set_norm(edhmc_ctx* ctx, const float*, const float*, const float*, const float*, void*) { return set_err(ctx, EDHMC_EUNSUPPORTED, "todo"); }
-int edhmc_sur_forward(edhmc_ctx* ctx, const float*, int32_t, float*, void*) { return set_err(ctx, EDHMC_EUNSUPPORTED, "todo"); }
-int edhmc_sur_loss_grad(edhmc_ctx* ctx, const float*, const float*, int32_t, double*, float*, void*) { return set_err(ctx, EDHMC_EUNSUPPORTED, "todo"); }
-int edhmc_sur_train_step(edhmc_ctx* ctx, const float*, const float*, int32_t, float, float, float, float, int32_t, double*, void*) { return set_err(ctx, EDHMC_EUNSUPPORTED, "todo"); }
-int edhmc_transition_surrogate(edhmc_ctx* ctx, float, int32_t, const float*, const double*, uint64_t, uint64_t, int32_t*, float*, int32_t*, void*) { return set_err(ctx, EDHMC_EUNSUPPORTED, "todo"); }
+#include "gemm_nt.h"
+#include "philox.h"
+
+namespace edhmc {
+
+using CfgM = TileCfg<2, 2, 2, 2>;   // 128 x 128 tiles for every MLP GEMM
+
+constexpr float kGeluK0 = 0.7978845608028654f;  // sqrt(2 / pi)
+constexpr float kGeluK1 = 0.044715f;
+
+__device__ inline float tanh_fast(float a) {
+  return 1.0f - 2.0f * fast_rcp(__expf(2.0f * a) + 1.0f);
 }
+
+// GELU(x) = x/2 (1 + tanh(k0 (x + k1 x^3)))  (Hendrycks & Gimpel 2016).
+__device__ inline float gelu(float x) {
+  const float t = tanh_fast(kGeluK0 * (x + kGeluK1 * x * x * x));
+  return 0.5f * x * (1.0f + t);
+}
+
+__device__ inline float gelu_grad(float x) {
+  const float t = tanh_fast(kGeluK0 * (x + kGeluK1 * x * x * x));
+  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * kGeluK0 * (1.0f + 3.0f * kGeluK1 * x * x);
+}
+
+struct Surrogate {
+  int D = 0, H = 0, Bmax = 0;
+  int Dp = 0, Hp = 0, Bp = 0;          // padded sizes
+  int Kc = 0;                          // output-layer input width Hp + Dp ([h2 | x])
+  int64_t n_params = 0;                // unpadded flat count
+  // padded fp32 masters, grads, Adam moments: [W1 Hp x Dp | b1 Hp | W2 Hp x Hp | b2 Hp |
+  //                                          W3 Dp x Kc | b3 Dp]
+  int64_t oW1, ob1, oW2, ob2, oW3, ob3, n_pad;
+  DevBuf P, Gr, Am, Av;
+  DevBuf W1b, W2b, W3b, W2t, W3t;     // bf16 copies: forward [out x in], backward [in x out]
+  DevBuf nrm;                          // in_shift, in_scale, out_shift, out_scale (each Dp)
+  DevBuf x0, a1, h1, a2, h2, o;        // activations (batch Bp); h2 is [Bp x Kc] = [h2 | x]
+  DevBuf dO, da2, da1, x0t, h1t, h2t, dOt, da2t, da1t;
+  DevBuf loss;                         // fp64 [1]
+  // sampling state
+  DevBuf ghat, ghat0;                  // [C x D_pad of the GLM]
+  bool ghat_valid = false;
+  bool norm_set = false;
+};
+
+void surrogate_invalidate(Surrogate* s) {
+  if (s) s->ghat_valid = false;
+}
+
+void surrogate_destroy(Surrogate* s) {
+  if (!s) return;
+  DevBuf* b[] = {&s->P, &s->Gr, &s->Am, &s->Av, &s->W1b, &s->W2b, &s->W3b, &s->W2t, &s->W3t,
+                 &s->nrm, &s->x0, &s->a1, &s->h1, &s->a2, &s->h2, &s->o, &s->dO, &s->da2,
+                 &s->da1, &s->x0t, &s->h1t, &s->h2t, &s->dOt, &s->da2t, &s->da1t, &s->loss,
+                 &s->ghat, &s->ghat0};
+  for (DevBuf* x : b) x->release();
+  delete s;
+}
+
+// ---------------------------------------------------------------------------
+// Epilogues
+// ---------------------------------------------------------------------------
+// a = acc + bias; store a (fp32, optional) and h = GELU(a) (bf16).
+struct EpiBiasGelu {
+  const float* bias;
+  float* a_out;      // [M x ld] or null
+  bf16_t* h_out;     // [M x ldh]
+  int64_t ld, ldh;
+  template <typename Cfg>
+  __device__ void apply(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p) const {
+#pragma unroll
+    for (int j = 0; j < Cfg::kWN; ++j) {
+      const int n = p.n0 + 32 * j + (p.lane & 31);
+      const float bj = bias[n];
+#pragma unroll
+      for (int i = 0; i < Cfg::kWM; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int64_t m = p.m0 + 32 * i + acc_row(r, p.lane);
+          const float a = acc[i][j][r] + bj;
+          if (a_out) a_out[m * ld + n] = a;
+          h_out[m * ldh + n] = f32_to_bf16(gelu(a));
+        }
+    }
+  }
+};
+
+// da = acc * GELU'(a) (bf16).
+struct EpiGeluBack {
+  const float* a_in;
+  bf16_t* da_out;
+  int64_t ld;
+  template <typename Cfg>
+  __device__ void apply(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p) const {
+#pragma unroll
+    for (int j = 0; j < Cfg::kWN; ++j) {
+      const int n = p.n0 + 32 * j + (p.lane & 31);
+#pragma unroll
+      for (int i = 0; i < Cfg::kWM; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int64_t m = p.m0 + 32 * i + acc_row(r, p.lane);
+          da_out[m * ld + n] = f32_to_bf16(acc[i][j][r] * gelu_grad(a_in[m * ld + n]));
+        }
+    }
+  }
+};
+
+// Training output: o = acc + b3; loss += (o - t)^2; dO = 2 (o - t) / (B D).
+struct EpiLoss {
+  const float* bias;
+  const float* G;            // targets [B x D] (raw gradients)
+  const float* out_shift; const float* out_scale;
+  int B, D;
+  int64_t ld;                // Dp
+  float coef;                // 2 / (B D)
+  bf16_t* dO;
+  double* loss;              // fp64 accumulator
+  template <typename Cfg>
+  __device__ void apply(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p) const {
+    double part = 0.0;
+#pragma unroll
+    for (int j = 0; j < Cfg::kWN; ++j) {
+      const int n = p.n0 + 32 * j + (p.lane & 31);
+      const float bj = bias[n];
+      const bool nv = n < D;
+      const float sh = nv ? out_shift[n] : 0.0f, isc = nv ? 1.0f / out_scale[n] : 0.0f;
+#pragma unroll
+      for (int i = 0; i < Cfg::kWM; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int m = p.m0 + 32 * i + acc_row(r, p.lane);
+          float d = 0.0f;
+          if (nv && m < B) {
+            const float t = (G[(int64_t)m * D + n] - sh) * isc;
+            d = acc[i][j][r] + bj - t;
+            part += (double)d * (double)d;
+          }
+          dO[(int64_t)m * ld + n] = f32_to_bf16(coef * d);
+        }
+    }
+    part = wave_sum_f64(part) * (0.5 * (double)coef);   // mean over B x D
+    if (p.lane == 0) atomicAdd(loss, part);
+  }
+};
+
+// Inference output: grad = (acc + b3) * out_scale + out_shift -> [B x D] fp32.
+struct EpiGradOut {
+  const float* bias; const float* out_shift; const float* out_scale;
+  int B, D;
+  float* out;                // [B x ldo]
+  int64_t ldo;
+  template <typename Cfg>
+  __device__ void apply(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p) const {
+#pragma unroll
+    for (int j = 0; j < Cfg::kWN; ++j) {
+      const int n = p.n0 + 32 * j + (p.lane & 31);
+      if (n >= D) continue;
+      const float bj = bias[n], sc = out_scale[n], sh = out_shift[n];
+#pragma unroll
+      for (int i = 0; i < Cfg::kWM; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int m = p.m0 + 32 * i + acc_row(r, p.lane);
+          if (m < B) out[(int64_t)m * ldo + n] = (acc[i][j][r] + bj) * sc + sh;
+        }
+    }
+  }
+};
+
+// Leapfrog step fused into the output layer: ghat = surrogate gradient at z;
+// middle steps kick twice by eps/2 and drift, and write the next normalised
+// input; the last step kicks by eps/2 and writes the hi/lo position planes
+// for the exact log joint.
+struct EpiLeapfrog {
+  const float* bias; const float* out_shift; const float* out_scale;
+  const float* in_shift; const float* in_scale;
+  int C, D, Dz;              // chains, dims, row stride of z/r/ghat
+  int Dp;                    // row stride of x0
+  float half, full;
+  int last;                  // 1: final half kick only
+  int eval_only;             // 1: store ghat only (no kick / drift)
+  float* z; float* r; float* ghat;
+  bf16_t* x0;                // next MLP input
+  bf16_t* xcat; int xcat_ld; // its copy in the output layer's [h2 | x] input
+  bf16_t* zlp; int64_t zlp_plane; int zlp_ld;   // hi/lo planes (last step)
+  template <typename Cfg>
+  __device__ void apply(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p) const {
+#pragma unroll
+    for (int j = 0; j < Cfg::kWN; ++j) {
+      const int n = p.n0 + 32 * j + (p.lane & 31);
+      if (n >= D) continue;
+      const float bj = bias[n], sc = out_scale[n], sh = out_shift[n];
+      const float ish = in_shift[n], isc = in_scale[n];
+#pragma unroll
+      for (int i = 0; i < Cfg::kWM; ++i)
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) {
+          const int m = p.m0 + 32 * i + acc_row(rr, p.lane);
+          if (m >= C) continue;
+          const int64_t idx = (int64_t)m * Dz + n;
+          const float g = (acc[i][j][rr] + bj) * sc + sh;
+          ghat[idx] = g;
+          if (eval_only) continue;
+          float rv = r[idx] + half * g;
+          if (last) {
+            r[idx] = rv;
+            const float zv = z[idx];
+            const bf16_t hi = f32_to_bf16(zv);
+            zlp[(int64_t)m * zlp_ld + n] = hi;
+            zlp[(int64_t)m * zlp_ld + n + zlp_plane] = f32_to_bf16(zv - bf16_to_f32(hi));
+          } else {
+            rv = rv + half * g;
+            const float zn = z[idx] + full * rv;
+            r[idx] = rv;
+            z[idx] = zn;
+            const bf16_t xb = f32_to_bf16((zn - ish) * isc);
+            x0[(int64_t)m * Dp + n] = xb;
+            xcat[(int64_t)m * xcat_ld + n] = xb;
+          }
+        }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Elementwise kernels
+// ---------------------------------------------------------------------------
+// x0 = bf16((w - in_shift) * in_scale) for rows < B, cols < D; zeros elsewhere.
+__global__ void norm_in_kernel(const float* __restrict__ W, int B, int D, int64_t ldw,
+                               const float* __restrict__ shift, const float* __restrict__ scale,
+                               bf16_t* __restrict__ x0, int Bp, int Dp, bf16_t* __restrict__ xcat,
+                               int xcat_ld) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)Bp * Dp) return;
+  const int b = (int)(i / Dp), d = (int)(i - (int64_t)b * Dp);
+  float v = 0.0f;
+  if (b < B && d < D) v = (W[(int64_t)b * ldw + d] - shift[d]) * scale[d];
+  const bf16_t xb = f32_to_bf16(v);
+  x0[i] = xb;
+  xcat[(int64_t)b * xcat_ld + d] = xb;
+}
+
+// out [C x R] = in [R x C]^T for bf16, 64 x 64 tiles (R, C multiples of 64).
+__global__ void __launch_bounds__(256)
+transpose_bf16_kernel(const bf16_t* __restrict__ in, int R, int Cc, bf16_t* __restrict__ out) {
+  __shared__ bf16_t t[64][66];
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int k = ty; k < 64; k += 4) t[k][tx] = in[(int64_t)(r0 + k) * Cc + c0 + tx];
+  __syncthreads();
+  for (int k = ty; k < 64; k += 4) out[(int64_t)(c0 + k) * R + r0 + tx] = t[tx][k];
+}
+
+// Row sums of a bf16 matrix [R x K] -> fp32 (bias gradients from transposed
+// activation gradients).  One wave per row.
+__global__ void rowsum_bf16_kernel(const bf16_t* __restrict__ in, int R, int K, float* __restrict__ out) {
+  const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= R) return;
+  float s = 0.0f;
+  for (int k = lane; k < K; k += 64) s += bf16_to_f32(in[(int64_t)row * K + k]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) out[row] = s;
+}
+
+// Adam (Kingma & Ba 2015) over the padded parameter vector.
+__global__ void adam_kernel(float* __restrict__ P, const float* __restrict__ G, float* __restrict__ M,
+                            float* __restrict__ V, int64_t n, float lr, float b1, float b2,
+                            float eps, float c1, float c2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float g = G[i];
+  const float m = b1 * M[i] + (1.0f - b1) * g;
+  const float v = b2 * V[i] + (1.0f - b2) * g * g;
+  M[i] = m;
+  V[i] = v;
+  P[i] -= lr * (m * c1) / (sqrtf(v * c2) + eps);
+}
+
+// fp32 [R x C] -> bf16 [R x C]
+__global__ void to_bf16_kernel(const float* __restrict__ in, int64_t n, bf16_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = f32_to_bf16(in[i]);
+}
+
+// Pack / unpack between the flat unpadded layout and the padded one.
+struct PackLayout {
+  int D, H, Dp, Hp;
+  int64_t oW1, ob1, oW2, ob2, oW3, ob3;
+};
+
+__device__ inline int64_t padded_index(const PackLayout& L, int64_t f) {
+  const int64_t sW1 = (int64_t)L.H * L.D, sW2 = (int64_t)L.H * L.H;
+  const int64_t sW3 = (int64_t)L.D * (L.H + L.D);
+  if (f < sW1) return L.oW1 + (f / L.D) * L.Dp + f % L.D;
+  f -= sW1;
+  if (f < L.H) return L.ob1 + f;
+  f -= L.H;
+  if (f < sW2) return L.oW2 + (f / L.H) * L.Hp + f % L.H;
+  f -= sW2;
+  if (f < L.H) return L.ob2 + f;
+  f -= L.H;
+  if (f < sW3) {   // W3 [D, H + D]: hidden columns then skip columns
+    const int64_t row = f / (L.H + L.D), col = f % (L.H + L.D);
+    return L.oW3 + row * (L.Hp + L.Dp) + (col < L.H ? col : L.Hp + (col - L.H));
+  }
+  f -= sW3;
+  return L.ob3 + f;
+}
+
+__global__ void pack_kernel(PackLayout L, int64_t n, const float* __restrict__ flat,
+                            float* __restrict__ padded, int to_padded) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  const int64_t pi = padded_index(L, f);
+  if (to_padded) padded[pi] = flat[f];
+  else ((float*)flat)[f] = padded[pi];
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+static inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+static PackLayout layout_of(const Surrogate* s) {
+  PackLayout L;
+  L.D = s->D; L.H = s->H; L.Dp = s->Dp; L.Hp = s->Hp;
+  L.oW1 = s->oW1; L.ob1 = s->ob1; L.oW2 = s->oW2; L.ob2 = s->ob2; L.oW3 = s->oW3; L.ob3 = s->ob3;
+  return L;
+}
+
+template <typename Epi>
+static int mlp_gemm(edhmc_ctx* ctx, const void* A, int64_t lda_elems, const void* B,
+                    int64_t ldb_elems, int M, int N, int K, Epi e, hipStream_t st) {
+  GemmShape g = {};
+  g.A = (const char*)A;
+  g.lda_b = lda_elems * 2;
+  g.a_stage_b = kStageBytes;
+  g.a_plane_b = 0;
+  g.B = (const char*)B;
+  g.ldb_b = ldb_elems * 2;
+  g.b_stage_b = kStageBytes;
+  g.m_tiles = M / CfgM::BM;
+  g.n_tiles = N / CfgM::BN;
+  g.k_stages = K * 2 / kStageBytes;
+  g.ksplit = 1;
+  g.stages_per_split = g.k_stages;
+  g.swz = 0;
+  ProfScope ps(ctx, PK_MLP, st);
+  hipLaunchKernelGGL((gemm_nt_kernel<bf16_t, CfgM, 1, Epi>), dim3(grid_size(g)),
+                     dim3(CfgM::kThreads), 0, st, g, e);
+  return check_launch(ctx, "mlp_gemm");
+}
+
+static int transpose(edhmc_ctx* ctx, const DevBuf& in, int R, int Cc, DevBuf& out, hipStream_t st) {
+  hipLaunchKernelGGL(transpose_bf16_kernel, dim3(Cc / 64, R / 64), dim3(256), 0, st,
+                     (const bf16_t*)in.p, R, Cc, (bf16_t*)out.p);
+  return check_launch(ctx, "transpose");
+}
+
+// Refresh the bf16 weight copies (forward and transposed) from the masters.
+static int refresh_weights(edhmc_ctx* ctx, Surrogate* s, hipStream_t st) {
+  const float* P = (const float*)s->P.p;
+  auto cvt = [&](int64_t off, int64_t n, DevBuf& out) {
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(n)), dim3(256), 0, st, P + off, n,
+                       (bf16_t*)out.p);
+    return check_launch(ctx, "to_bf16");
+  };
+  EDHMC_TRY(cvt(s->oW1, (int64_t)s->Hp * s->Dp, s->W1b));
+  EDHMC_TRY(cvt(s->oW2, (int64_t)s->Hp * s->Hp, s->W2b));
+  EDHMC_TRY(cvt(s->oW3, (int64_t)s->Dp * s->Kc, s->W3b));
+  EDHMC_TRY(transpose(ctx, s->W2b, s->Hp, s->Hp, s->W2t, st));
+  EDHMC_TRY(transpose(ctx, s->W3b, s->Dp, s->Kc, s->W3t, st));
+  s->ghat_valid = false;
+  return EDHMC_OK;
+}
+
+// Forward through the two hidden layers for the batch in s->x0 (Bp rows);
+// keep_a: store pre-activations for the backward pass.
+static int forward_hidden(edhmc_ctx* ctx, Surrogate* s, int Bp, bool keep_a, hipStream_t st) {
+  const float* P = (const float*)s->P.p;
+  EpiBiasGelu e1;
+  e1.bias = P + s->ob1;
+  e1.a_out = keep_a ? (float*)s->a1.p : nullptr;
+  e1.h_out = (bf16_t*)s->h1.p;
+  e1.ld = s->Hp;
+  e1.ldh = s->Hp;
+  EDHMC_TRY(mlp_gemm(ctx, s->x0.p, s->Dp, s->W1b.p, s->Dp, Bp, s->Hp, s->Dp, e1, st));
+  EpiBiasGelu e2;
+  e2.bias = P + s->ob2;
+  e2.a_out = keep_a ? (float*)s->a2.p : nullptr;
+  e2.h_out = (bf16_t*)s->h2.p;
+  e2.ld = s->Hp;
+  e2.ldh = s->Kc;
+  return mlp_gemm(ctx, s->h1.p, s->Hp, s->W2b.p, s->Hp, Bp, s->Hp, s->Hp, e2, st);
+}
+
+static int check_sur(edhmc_ctx* ctx) {
+  if (!ctx->sur) return set_err(ctx, EDHMC_ESTATE, "no surrogate: call edhmc_sur_create");
+  return EDHMC_OK;
+}
+
+static int batch_ok(edhmc_ctx* ctx, Surrogate* s, int B) {
+  if (B <= 0 || B > s->Bmax)
+    return set_err(ctx, EDHMC_EINVAL, "batch %d outside 1..%d (max_batch)", B, s->Bmax);
+  if (!s->norm_set) return set_err(ctx, EDHMC_ESTATE, "normalisation not set");
+  return EDHMC_OK;
+}
+
+static int loss_grad(edhmc_ctx* ctx, Surrogate* s, const float* W, const float* G, int B,
+                     hipStream_t st) {
+  const int Bp = (int)round_up(B, 128);
+  const float* P = (const float*)s->P.p;
+  const float* nrm = (const float*)s->nrm.p;
+  hipLaunchKernelGGL(norm_in_kernel, dim3(blocks_for((int64_t)Bp * s->Dp)), dim3(256), 0, st, W,
+                     B, s->D, (int64_t)s->D, nrm, nrm + s->Dp, (bf16_t*)s->x0.p, Bp, s->Dp,
+                     (bf16_t*)s->h2.p + s->Hp, s->Kc);
+  EDHMC_TRY(check_launch(ctx, "norm_in"));
+  EDHMC_TRY(forward_hidden(ctx, s, Bp, true, st));
+  if (hipMemsetAsync(s->loss.p, 0, 8, st) != hipSuccess)
+    return set_err(ctx, EDHMC_EHIP, "memset loss");
+  EpiLoss el;
+  el.bias = P + s->ob3;
+  el.G = G;
+  el.out_shift = nrm + 2 * s->Dp;
+  el.out_scale = nrm + 3 * s->Dp;
+  el.B = B;
+  el.D = s->D;
+  el.ld = s->Dp;
+  el.coef = 2.0f / ((float)B * (float)s->D);
+  el.dO = (bf16_t*)s->dO.p;
+  el.loss = (double*)s->loss.p;
+  EDHMC_TRY(mlp_gemm(ctx, s->h2.p, s->Kc, s->W3b.p, s->Kc, Bp, s->Dp, s->Kc, el, st));
+  float* Gr = (float*)s->Gr.p;
+  // layer 3 weights: dW3 = dO^T h2, db3 = column sums of dO
+  EDHMC_TRY(transpose(ctx, s->dO, Bp, s->Dp, s->dOt, st));
+  EDHMC_TRY(transpose(ctx, s->h2, Bp, s->Kc, s->h2t, st));
+  EpiStoreF32 ew;
+  ew.split_b = 0;
+  ew.out = Gr + s->oW3;
+  ew.ldc = s->Kc;
+  EDHMC_TRY(mlp_gemm(ctx, s->dOt.p, Bp, s->h2t.p, Bp, s->Dp, s->Kc, Bp, ew, st));
+  hipLaunchKernelGGL(rowsum_bf16_kernel, dim3(blocks_for(s->Dp, 4)), dim3(256), 0, st,
+                     (const bf16_t*)s->dOt.p, s->Dp, Bp, Gr + s->ob3);
+  EDHMC_TRY(check_launch(ctx, "rowsum"));
+  // da2 = (dO W3) * GELU'(a2)
+  EpiGeluBack eb2;
+  eb2.a_in = (const float*)s->a2.p;
+  eb2.da_out = (bf16_t*)s->da2.p;
+  eb2.ld = s->Hp;
+  EDHMC_TRY(mlp_gemm(ctx, s->dO.p, s->Dp, s->W3t.p, s->Dp, Bp, s->Hp, s->Dp, eb2, st));
+  EDHMC_TRY(transpose(ctx, s->da2, Bp, s->Hp, s->da2t, st));
+  EDHMC_TRY(transpose(ctx, s->h1, Bp, s->Hp, s->h1t, st));
+  ew.out = Gr + s->oW2;
+  ew.ldc = s->Hp;
+  EDHMC_TRY(mlp_gemm(ctx, s->da2t.p, Bp, s->h1t.p, Bp, s->Hp, s->Hp, Bp, ew, st));
+  hipLaunchKernelGGL(rowsum_bf16_kernel, dim3(blocks_for(s->Hp, 4)), dim3(256), 0, st,
+                     (const bf16_t*)s->da2t.p, s->Hp, Bp, Gr + s->ob2);
+  EDHMC_TRY(check_launch(ctx, "rowsum"));
+  // da1 = (da2 W2) * GELU'(a1)
+  EpiGeluBack eb1;
+  eb1.a_in = (const float*)s->a1.p;
+  eb1.da_out = (bf16_t*)s->da1.p;
+  eb1.ld = s->Hp;
+  EDHMC_TRY(mlp_gemm(ctx, s->da2.p, s->Hp, s->W2t.p, s->Hp, Bp, s->Hp, s->Hp, eb1, st));
+  EDHMC_TRY(transpose(ctx, s->da1, Bp, s->Hp, s->da1t, st));
+  EDHMC_TRY(transpose(ctx, s->x0, Bp, s->Dp, s->x0t, st));
+  ew.out = Gr + s->oW1;
+  ew.ldc = s->Dp;
+  EDHMC_TRY(mlp_gemm(ctx, s->da1t.p, Bp, s->x0t.p, Bp, s->Hp, s->Dp, Bp, ew, st));
+  hipLaunchKernelGGL(rowsum_bf16_kernel, dim3(blocks_for(s->Hp, 4)), dim3(256), 0, st,
+                     (const bf16_t*)s->da1t.p, s->Hp, Bp, Gr + s->ob1);
+  return check_launch(ctx, "rowsum");
+}
+
+// Surrogate gradient at the chains' current positions into s->ghat.
+static int eval_ghat(edhmc_ctx* ctx, Surrogate* s, hipStream_t st);
+
+// Per-chain begin of a surrogate trajectory.
+struct SurChainArgs {
+  int C, D, Dz, Dp;
+  float* z; float* r; float* z0; float* ghat; float* ghat0;
+  double* K0;
+  const float* r_fed;
+  uint64_t seed, counter;
+  float half, full;
+  const float* in_shift; const float* in_scale;
+  bf16_t* x0;
+  bf16_t* xcat; int xcat_ld;
+};
+
+__global__ void __launch_bounds__(256) sur_begin_kernel(SurChainArgs a) {
+  __shared__ double red[4];
+  const int c = blockIdx.x;
+  double k = 0.0;
+  const int nb = (a.D + 3) / 4;
+  for (int j = threadIdx.x; j < nb; j += 256) {
+    f32x4_t rv;
+    if (a.r_fed) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int d = 4 * j + e;
+        rv[e] = d < a.D ? a.r_fed[(int64_t)c * a.D + d] : 0.0f;
+      }
+    } else {
+      rv = momentum4(a.seed, a.counter, (uint32_t)j, (uint32_t)c);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int d = 4 * j + e;
+      if (d >= a.D) continue;
+      const int64_t idx = (int64_t)c * a.Dz + d;
+      float r = rv[e];
+      k += (double)r * (double)r;
+      const float zv = a.z[idx], gv = a.ghat[idx];
+      a.z0[idx] = zv;
+      a.ghat0[idx] = gv;
+      r = r + a.half * gv;
+      const float zn = zv + a.full * r;
+      a.z[idx] = zn;
+      a.r[idx] = r;
+      const bf16_t xb = f32_to_bf16((zn - a.in_shift[d]) * a.in_scale[d]);
+      a.x0[(int64_t)c * a.Dp + d] = xb;
+      a.xcat[(int64_t)c * a.xcat_ld + d] = xb;
+    }
+  }
+  k = block_sum_f64<256>(k, red);
+  if (threadIdx.x == 0) a.K0[c] = 0.5 * k;
+}
+
+struct SurAcceptArgs {
+  int C, D, Dz;
+  float* z; float* r; float* z0; float* ghat; float* ghat0;
+  double* logp; const double* K0; const double* ll;
+  const float* ploc; const float* pscale;
+  const double* u_fed; uint64_t seed, counter;
+  float* sample_out; int32_t* accepted_out; int32_t* n_accept; int32_t* ndiv;
+  double* mom_mean; double* mom_m2; int64_t mom_n;
+};
+
+__global__ void __launch_bounds__(256) sur_accept_kernel(SurAcceptArgs a) {
+  __shared__ double red[4];
+  __shared__ int acc_flag;
+  const int c = blockIdx.x;
+  double kl = 0.0, plp = 0.0;
+  int bad = 0;
+  for (int d = threadIdx.x; d < a.D; d += 256) {
+    const int64_t idx = (int64_t)c * a.Dz + d;
+    const float zv = a.z[idx], rv = a.r[idx];
+    kl += (double)rv * (double)rv;
+    const double s = a.pscale[d];
+    const double t = ((double)zv - (double)a.ploc[d]) / s;
+    plp += -0.5 * t * t - log(s) - 0.5 * kLog2Pi;
+    if (!isfinite(zv) || !isfinite(rv)) bad = 1;
+  }
+  kl = block_sum_f64<256>(kl, red);
+  plp = block_sum_f64<256>(plp, red);
+  const double nbad = block_sum_f64<256>((double)bad, red);
+  if (threadIdx.x == 0) {
+    const double logpL = a.ll[c] + plp;
+    const double ratio = a.K0[c] - 0.5 * kl + logpL - a.logp[c];
+    const double u = a.u_fed ? a.u_fed[c] : accept_uniform(a.seed, a.counter, (uint32_t)c);
+    const int acc = (log(u) < ratio) && nbad == 0.0;
+    acc_flag = acc;
+    if (acc) a.logp[c] = logpL;
+    if (a.accepted_out) a.accepted_out[c] = acc;
+    if (a.n_accept) a.n_accept[c] += acc;
+    if (a.ndiv && (nbad != 0.0 || !(ratio == ratio))) a.ndiv[c] += 1;
+  }
+  __syncthreads();
+  const int acc = acc_flag;
+  for (int d = threadIdx.x; d < a.D; d += 256) {
+    const int64_t idx = (int64_t)c * a.Dz + d;
+    float zv;
+    if (acc) {
+      zv = a.z[idx];
+    } else {
+      zv = a.z0[idx];
+      a.z[idx] = zv;
+      a.ghat[idx] = a.ghat0[idx];
+    }
+    if (a.sample_out) a.sample_out[(int64_t)c * a.D + d] = zv;
+    if (a.mom_mean) {
+      const double x = zv;
+      const double mo = a.mom_mean[idx];
+      const double mn = mo + (x - mo) / (double)a.mom_n;
+      a.mom_mean[idx] = mn;
+      a.mom_m2[idx] += (x - mo) * (x - mn);
+    }
+  }
+}
+
+static EpiLeapfrog lf_epi(edhmc_ctx* ctx, Surrogate* s, float half, float full, int last,
+                          int eval_only) {
+  const float* P = (const float*)s->P.p;
+  const float* nrm = (const float*)s->nrm.p;
+  EpiLeapfrog e;
+  e.bias = P + s->ob3;
+  e.in_shift = nrm;
+  e.in_scale = nrm + s->Dp;
+  e.out_shift = nrm + 2 * s->Dp;
+  e.out_scale = nrm + 3 * s->Dp;
+  e.C = ctx->C;
+  e.D = ctx->D;
+  e.Dz = ctx->D_pad;
+  e.Dp = s->Dp;
+  e.half = half;
+  e.full = full;
+  e.last = last;
+  e.eval_only = eval_only;
+  e.z = (float*)ctx->z.p;
+  e.r = (float*)ctx->r.p;
+  e.ghat = (float*)s->ghat.p;
+  e.x0 = (bf16_t*)s->x0.p;
+  e.xcat = (bf16_t*)s->h2.p + s->Hp;
+  e.xcat_ld = s->Kc;
+  e.zlp = (bf16_t*)ctx->z_lp.p;
+  e.zlp_plane = (int64_t)ctx->C_pad * ctx->D_pad;
+  e.zlp_ld = ctx->D_pad;
+  return e;
+}
+
+static int sur_state_buffers(edhmc_ctx* ctx, Surrogate* s) {
+  if (!ctx->bound || !ctx->state_valid)
+    return set_err(ctx, EDHMC_ESTATE, "no chain state: call edhmc_set_state first");
+  if (ctx->cdt != EDHMC_BF16)
+    return set_err(ctx, EDHMC_EUNSUPPORTED, "surrogate transitions need a bf16-bound model");
+  if (ctx->D != s->D) return set_err(ctx, EDHMC_EINVAL, "surrogate D=%d, model D=%d", s->D, ctx->D);
+  if (ctx->C > s->Bmax)
+    return set_err(ctx, EDHMC_EINVAL, "%d chains exceed the surrogate batch %d", ctx->C, s->Bmax);
+  const size_t bytes = (size_t)ctx->C * ctx->D_pad * 4;
+  if (s->ghat.bytes != bytes) {
+    EDHMC_TRY(dev_alloc(ctx, s->ghat, bytes));
+    EDHMC_TRY(dev_alloc(ctx, s->ghat0, bytes));
+    s->ghat_valid = false;
+  }
+  return EDHMC_OK;
+}
+
+static int eval_ghat(edhmc_ctx* ctx, Surrogate* s, hipStream_t st) {
+  const int Bp = (int)round_up(ctx->C, 128);
+  const float* nrm = (const float*)s->nrm.p;
+  hipLaunchKernelGGL(norm_in_kernel, dim3(blocks_for((int64_t)Bp * s->Dp)), dim3(256), 0, st,
+                     (const float*)ctx->z.p, ctx->C, s->D, (int64_t)ctx->D_pad, nrm, nrm + s->Dp,
+                     (bf16_t*)s->x0.p, Bp, s->Dp,
+                     (bf16_t*)s->h2.p + s->Hp, s->Kc);
+  EDHMC_TRY(check_launch(ctx, "norm_in"));
+  EDHMC_TRY(forward_hidden(ctx, s, Bp, false, st));
+  EpiLeapfrog e = lf_epi(ctx, s, 0.0f, 0.0f, 0, 1);
+  EDHMC_TRY(mlp_gemm(ctx, s->h2.p, s->Kc, s->W3b.p, s->Kc, Bp, s->Dp, s->Kc, e, st));
+  s->ghat_valid = true;
+  return EDHMC_OK;
+}
+
+static int sur_transition(edhmc_ctx* ctx, float eps, int L, const float* r0, const double* u,
+                          uint64_t seed, uint64_t counter, int32_t* accepted_out,
+                          float* sample_out, int32_t* n_accept, hipStream_t st) {
+  Surrogate* s = ctx->sur;
+  EDHMC_TRY(sur_state_buffers(ctx, s));
+  if (L < 1) return set_err(ctx, EDHMC_EINVAL, "n_steps must be >= 1, got %d", L);
+  if (!std::isfinite(eps)) return set_err(ctx, EDHMC_EINVAL, "step_size must be finite");
+  if (!s->norm_set) return set_err(ctx, EDHMC_ESTATE, "normalisation not set");
+  if (!s->ghat_valid) EDHMC_TRY(eval_ghat(ctx, s, st));
+  const float half = (float)(0.5 * (double)eps), full = eps;
+  const float* nrm = (const float*)s->nrm.p;
+  SurChainArgs a;
+  a.C = ctx->C; a.D = ctx->D; a.Dz = ctx->D_pad; a.Dp = s->Dp;
+  a.z = (float*)ctx->z.p; a.r = (float*)ctx->r.p; a.z0 = (float*)ctx->z0.p;
+  a.ghat = (float*)s->ghat.p; a.ghat0 = (float*)s->ghat0.p;
+  a.K0 = (double*)ctx->K0.p;
+  a.r_fed = r0; a.seed = seed; a.counter = counter;
+  a.half = half; a.full = full;
+  a.in_shift = nrm; a.in_scale = nrm + s->Dp;
+  a.x0 = (bf16_t*)s->x0.p;
+  a.xcat = (bf16_t*)s->h2.p + s->Hp;
+  a.xcat_ld = s->Kc;
+  {
+    ProfScope ps(ctx, PK_CHAIN, st);
+    hipLaunchKernelGGL(sur_begin_kernel, dim3(ctx->C), dim3(256), 0, st, a);
+  }
+  EDHMC_TRY(check_launch(ctx, "sur_begin"));
+  const int Bp = (int)round_up(ctx->C, 128);
+  for (int k = 1; k <= L; ++k) {
+    EDHMC_TRY(forward_hidden(ctx, s, Bp, false, st));
+    EpiLeapfrog e = lf_epi(ctx, s, half, full, k == L ? 1 : 0, 0);
+    EDHMC_TRY(mlp_gemm(ctx, s->h2.p, s->Kc, s->W3b.p, s->Kc, Bp, s->Dp, s->Kc, e, st));
+  }
+  // exact log joint at the end point (z_lp planes written by the last step)
+  EDHMC_TRY(glm_forward_ll(ctx, st));
+  SurAcceptArgs b;
+  b.C = ctx->C; b.D = ctx->D; b.Dz = ctx->D_pad;
+  b.z = (float*)ctx->z.p; b.r = (float*)ctx->r.p; b.z0 = (float*)ctx->z0.p;
+  b.ghat = (float*)s->ghat.p; b.ghat0 = (float*)s->ghat0.p;
+  b.logp = (double*)ctx->logp.p; b.K0 = (const double*)ctx->K0.p;
+  b.ll = (const double*)ctx->ll.p;
+  b.ploc = (const float*)ctx->ploc.p; b.pscale = (const float*)ctx->pscale.p;
+  b.u_fed = u; b.seed = seed; b.counter = counter;
+  b.sample_out = sample_out; b.accepted_out = accepted_out; b.n_accept = n_accept;
+  b.ndiv = (int32_t*)ctx->ndiv.p;
+  b.mom_mean = nullptr; b.mom_m2 = nullptr; b.mom_n = 0;
+  if (ctx->mom_on) {
+    ctx->mom_count += 1;
+    b.mom_mean = (double*)ctx->mom_mean.p;
+    b.mom_m2 = (double*)ctx->mom_m2.p;
+    b.mom_n = ctx->mom_count;
+  }
+  {
+    ProfScope ps(ctx, PK_CHAIN, st);
+    hipLaunchKernelGGL(sur_accept_kernel, dim3(ctx->C), dim3(256), 0, st, b);
+  }
+  EDHMC_TRY(check_launch(ctx, "sur_accept"));
+  // the exact gradient at the new state is not known any more
+  ctx->g_valid = false;
+  return EDHMC_OK;
+}
+
+}  // namespace edhmc
+
+using namespace edhmc;
+
+extern "C" {
+
+int edhmc_sur_create(edhmc_ctx* ctx, int32_t D, int32_t H, int32_t max_batch) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  if (D <= 0 || H <= 0 || max_batch <= 0 || D > 16384 || H > 16384)
+    return set_err(ctx, EDHMC_EINVAL, "bad surrogate sizes D=%d H=%d batch=%d", D, H, max_batch);
+  if (ctx->sur) surrogate_destroy(ctx->sur);
+  Surrogate* s = new (std::nothrow) Surrogate();
+  if (!s) return set_err(ctx, EDHMC_ENOMEM, "host allocation");
+  ctx->sur = s;
+  s->D = D; s->H = H; s->Bmax = max_batch;
+  s->Dp = (int)round_up(D, 128);
+  s->Hp = (int)round_up(H, 128);
+  s->Bp = (int)round_up(max_batch, 128);
+  s->Kc = s->Hp + s->Dp;
+  s->n_params = (int64_t)H * D + H + (int64_t)H * H + H + (int64_t)D * (H + D) + D;
+  s->oW1 = 0;
+  s->ob1 = s->oW1 + (int64_t)s->Hp * s->Dp;
+  s->oW2 = s->ob1 + s->Hp;
+  s->ob2 = s->oW2 + (int64_t)s->Hp * s->Hp;
+  s->oW3 = s->ob2 + s->Hp;
+  s->ob3 = s->oW3 + (int64_t)s->Dp * s->Kc;
+  s->n_pad = s->ob3 + s->Dp;
+  const size_t pb = (size_t)s->n_pad * 4;
+  EDHMC_TRY(dev_alloc(ctx, s->P, pb));
+  EDHMC_TRY(dev_alloc(ctx, s->Gr, pb));
+  EDHMC_TRY(dev_alloc(ctx, s->Am, pb));
+  EDHMC_TRY(dev_alloc(ctx, s->Av, pb));
+  const size_t hd = (size_t)s->Hp * s->Dp * 2, hh = (size_t)s->Hp * s->Hp * 2;
+  EDHMC_TRY(dev_alloc(ctx, s->W1b, hd));
+  EDHMC_TRY(dev_alloc(ctx, s->W2b, hh));
+  EDHMC_TRY(dev_alloc(ctx, s->W3b, (size_t)s->Dp * s->Kc * 2));
+  EDHMC_TRY(dev_alloc(ctx, s->W2t, hh));
+  EDHMC_TRY(dev_alloc(ctx, s->W3t, (size_t)s->Dp * s->Kc * 2));
+  EDHMC_TRY(dev_alloc(ctx, s->nrm, (size_t)4 * s->Dp * 4));
+  const size_t bd = (size_t)s->Bp * s->Dp, bh = (size_t)s->Bp * s->Hp;
+  EDHMC_TRY(dev_alloc(ctx, s->x0, bd * 2));
+  EDHMC_TRY(dev_alloc(ctx, s->a1, bh * 4));
+  EDHMC_TRY(dev_alloc(ctx, s->h1, bh * 2));
+  EDHMC_TRY(dev_alloc(ctx, s->a2, bh * 4));
+  EDHMC_TRY(dev_alloc(ctx, s->h2, (size_t)s->Bp * s->Kc * 2));
+  EDHMC_TRY(dev_alloc(ctx, s->dO, bd * 2));
+  EDHMC_TRY(dev_alloc(ctx, s->da2, bh * 2));
+  EDHMC_TRY(dev_alloc(ctx, s->da1, bh * 2));
+  EDHMC_TRY(dev_alloc(ctx, s->x0t, bd * 2));
+  EDHMC_TRY(dev_alloc(ctx, s->h1t, bh * 2));
+  EDHMC_TRY(dev_alloc(ctx, s->h2t, (size_t)s->Bp * s->Kc * 2));
+  EDHMC_TRY(dev_alloc(ctx, s->dOt, bd * 2));
+  EDHMC_TRY(dev_alloc(ctx, s->da2t, bh * 2));
+  EDHMC_TRY(dev_alloc(ctx, s->da1t, bh * 2));
+  EDHMC_TRY(dev_alloc(ctx, s->loss, 8));
+  return EDHMC_OK;
+}
+
+int64_t edhmc_sur_num_params(edhmc_ctx* ctx) {
+  if (!ctx || !ctx->sur) return -1;
+  return ctx->sur->n_params;
+}
+
+int edhmc_sur_set_params(edhmc_ctx* ctx, const float* params, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  EDHMC_TRY(check_sur(ctx));
+  Surrogate* s = ctx->sur;
+  hipStream_t st = (hipStream_t)stream;
+  if (!params) return set_err(ctx, EDHMC_EINVAL, "params is NULL");
+  if (hipMemsetAsync(s->P.p, 0, s->P.bytes, st) != hipSuccess ||
+      hipMemsetAsync(s->Am.p, 0, s->Am.bytes, st) != hipSuccess ||
+      hipMemsetAsync(s->Av.p, 0, s->Av.bytes, st) != hipSuccess)
+    return set_err(ctx, EDHMC_EHIP, "memset");
+  hipLaunchKernelGGL(pack_kernel, dim3(blocks_for(s->n_params)), dim3(256), 0, st, layout_of(s),
+                     s->n_params, params, (float*)s->P.p, 1);
+  EDHMC_TRY(check_launch(ctx, "pack"));
+  return refresh_weights(ctx, s, st);
+}
+
+int edhmc_sur_get_params(edhmc_ctx* ctx, float* params_out, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  EDHMC_TRY(check_sur(ctx));
+  Surrogate* s = ctx->sur;
+  if (!params_out) return set_err(ctx, EDHMC_EINVAL, "params_out is NULL");
+  hipLaunchKernelGGL(pack_kernel, dim3(blocks_for(s->n_params)), dim3(256), 0,
+                     (hipStream_t)stream, layout_of(s), s->n_params, params_out, (float*)s->P.p,
+                     0);
+  return check_launch(ctx, "unpack");
+}
+
+int edhmc_sur_set_norm(edhmc_ctx* ctx, const float* in_shift, const float* in_scale,
+                       const float* out_shift, const float* out_scale, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  EDHMC_TRY(check_sur(ctx));
+  Surrogate* s = ctx->sur;
+  hipStream_t st = (hipStream_t)stream;
+  if (!in_shift || !in_scale || !out_shift || !out_scale)
+    return set_err(ctx, EDHMC_EINVAL, "NULL normalisation vector");
+  float* n = (float*)s->nrm.p;
+  if (hipMemsetAsync(n, 0, s->nrm.bytes, st) != hipSuccess)
+    return set_err(ctx, EDHMC_EHIP, "memset");
+  const float* src[4] = {in_shift, in_scale, out_shift, out_scale};
+  for (int k = 0; k < 4; ++k)
+    if (hipMemcpyAsync(n + k * s->Dp, src[k], (size_t)s->D * 4, hipMemcpyDeviceToDevice, st) !=
+        hipSuccess)
+      return set_err(ctx, EDHMC_EHIP, "copy normalisation");
+  s->norm_set = true;
+  s->ghat_valid = false;
+  return EDHMC_OK;
+}
+
+int edhmc_sur_forward(edhmc_ctx* ctx, const float* W, int32_t B, float* grad_out, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  EDHMC_TRY(check_sur(ctx));
+  Surrogate* s = ctx->sur;
+  EDHMC_TRY(batch_ok(ctx, s, B));
+  if (!W || !grad_out) return set_err(ctx, EDHMC_EINVAL, "NULL argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int Bp = (int)round_up(B, 128);
+  const float* nrm = (const float*)s->nrm.p;
+  hipLaunchKernelGGL(norm_in_kernel, dim3(blocks_for((int64_t)Bp * s->Dp)), dim3(256), 0, st, W,
+                     B, s->D, (int64_t)s->D, nrm, nrm + s->Dp, (bf16_t*)s->x0.p, Bp, s->Dp,
+                     (bf16_t*)s->h2.p + s->Hp, s->Kc);
+  EDHMC_TRY(check_launch(ctx, "norm_in"));
+  EDHMC_TRY(forward_hidden(ctx, s, Bp, false, st));
+  EpiGradOut e;
+  e.bias = (const float*)s->P.p + s->ob3;
+  e.out_shift = nrm + 2 * s->Dp;
+  e.out_scale = nrm + 3 * s->Dp;
+  e.B = B;
+  e.D = s->D;
+  e.out = grad_out;
+  e.ldo = s->D;
+  s->ghat_valid = false;   // x0/h buffers reused
+  return mlp_gemm(ctx, s->h2.p, s->Kc, s->W3b.p, s->Kc, Bp, s->Dp, s->Kc, e, st);
+}
+
+int edhmc_sur_loss_grad(edhmc_ctx* ctx, const float* W, const float* G, int32_t B, double* loss_out,
+                        float* param_grad_out, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  EDHMC_TRY(check_sur(ctx));
+  Surrogate* s = ctx->sur;
+  EDHMC_TRY(batch_ok(ctx, s, B));
+  if (!W || !G) return set_err(ctx, EDHMC_EINVAL, "NULL argument");
+  hipStream_t st = (hipStream_t)stream;
+  s->ghat_valid = false;
+  EDHMC_TRY(loss_grad(ctx, s, W, G, B, st));
+  if (loss_out && hipMemcpyAsync(loss_out, s->loss.p, 8, hipMemcpyDeviceToDevice, st) != hipSuccess)
+    return set_err(ctx, EDHMC_EHIP, "copy loss");
+  if (param_grad_out) {
+    hipLaunchKernelGGL(pack_kernel, dim3(blocks_for(s->n_params)), dim3(256), 0, st, layout_of(s),
+                       s->n_params, param_grad_out, (float*)s->Gr.p, 0);
+    EDHMC_TRY(check_launch(ctx, "unpack"));
+  }
+  return EDHMC_OK;
+}
+
+int edhmc_sur_train_step(edhmc_ctx* ctx, const float* W, const float* G, int32_t B, float lr,
+                         float beta1, float beta2, float adam_eps, int32_t step, double* loss_out,
+                         void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  EDHMC_TRY(check_sur(ctx));
+  Surrogate* s = ctx->sur;
+  if (step < 1) return set_err(ctx, EDHMC_EINVAL, "Adam step count starts at 1");
+  EDHMC_TRY(edhmc_sur_loss_grad(ctx, W, G, B, loss_out, nullptr, stream));
+  hipStream_t st = (hipStream_t)stream;
+  const float c1 = (float)(1.0 / (1.0 - std::pow((double)beta1, step)));
+  const float c2 = (float)(1.0 / (1.0 - std::pow((double)beta2, step)));
+  hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(s->n_pad)), dim3(256), 0, st, (float*)s->P.p,
+                     (const float*)s->Gr.p, (float*)s->Am.p, (float*)s->Av.p, s->n_pad, lr, beta1,
+                     beta2, adam_eps, c1, c2);
+  EDHMC_TRY(check_launch(ctx, "adam"));
+  return refresh_weights(ctx, s, st);
+}
+
+int edhmc_transition_surrogate(edhmc_ctx* ctx, float eps, int32_t n_steps, const float* r0,
+                               const double* u, uint64_t seed, uint64_t counter,
+                               int32_t* accepted_out, float* sample_out, int32_t* n_accept,
+                               void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  EDHMC_TRY(check_sur(ctx));
+  return sur_transition(ctx, eps, n_steps, r0, u, seed, counter, accepted_out, sample_out,
+                        n_accept, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -167,6 +167,26 @@ class GLMEngine(Context):
     self.check(self.lib.edhmc_divergences(self.h, L.ptr(out), self.stream))
     return out
 
+  def record_pairs(self, W_buf=None, G_buf=None):
+    """Record (position, exact gradient) at every inner leapfrog point of the
+    next plain transitions into W_buf/G_buf [capacity, D]; None stops."""
+    cap = 0 if W_buf is None else int(W_buf.shape[0])
+    self._rec = (W_buf, G_buf)   # keep the buffers alive while recording
+    self.check(self.lib.edhmc_record_pairs(self.h, L.ptr(W_buf), L.ptr(G_buf), cap))
+
+  def record_count(self):
+    return int(self.lib.edhmc_record_count(self.h))
+
+  def col_stats(self, X):
+    """Column mean and standard deviation (divisor P) of X [P, D] on device."""
+    X = X.contiguous()
+    P, D = X.shape
+    mean = torch.empty(D, device=self.device)
+    sd = torch.empty(D, device=self.device)
+    self.check(self.lib.edhmc_col_stats(self.h, L.ptr(X), int(P), int(D), L.ptr(mean), L.ptr(sd),
+                                        self.stream))
+    return mean, sd
+
   def draw_momenta(self, C, seed, counter):
     out = torch.empty(C, self.D, device=self.device)
     self.check(self.lib.edhmc_draw_momenta(self.h, C, self.D, int(seed), int(counter),

@@ -1,0 +1,197 @@
+"""Learned gradient surrogate for HMC (no counterpart in the reference).
+
+An MLP D -> H -> H -> D (tanh-GELU) approximates grad log p(w | data).  It is
+trained on the GPU by libedhmc.so's own forward/backward/Adam kernels on
+(w, grad) pairs that the plain sampler records during warm-up, then frozen and
+used as the leapfrog force; the exact log joint (bf16 X, hi/lo split
+positions, FP64 reduction) still decides every Metropolis test, so the chain
+targets the exact posterior.  A frozen surrogate keeps the kernel a fixed,
+reversible, volume-preserving proposal; adapting it during sampling would
+break the Markov property.
+"""
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from edward_amd import _lib as L
+
+
+class Surrogate(object):
+  """Gradient surrogate attached to a bf16 ``GLMEngine``.
+
+  Args:
+    engine: the GLMEngine whose posterior is sampled (precision 'bf16').
+    hidden: hidden width H.
+    max_batch: largest batch for training / number of chains.
+    seed: seed of the weight initialisation (Glorot-uniform).
+  """
+
+  def __init__(self, engine, hidden=1024, max_batch=4096, seed=0):
+    self.engine = engine
+    self.D = engine.D
+    self.H = int(hidden)
+    self.max_batch = int(max_batch)
+    engine.check(engine.lib.edhmc_sur_create(engine.h, self.D, self.H, self.max_batch))
+    self.n_params = int(engine.lib.edhmc_sur_num_params(engine.h))
+    self.step_count = 0
+    self.init_params(seed)
+
+  # -- parameters ----------------------------------------------------------
+  def init_params(self, seed=0):
+    rng = np.random.default_rng(seed)
+    D, H = self.D, self.H
+
+    def glorot(o, i):
+      a = math.sqrt(6.0 / (i + o))
+      return rng.uniform(-a, a, size=(o, i))
+    # output layer [hidden | skip]: small hidden part, zero skip (fit() starts
+    # the skip at the least-squares linear map)
+    W3 = np.concatenate([0.1 * glorot(D, H), np.zeros((D, D))], axis=1)
+    flat = np.concatenate([glorot(H, D).ravel(), np.zeros(H), glorot(H, H).ravel(), np.zeros(H),
+                           W3.ravel(), np.zeros(D)]).astype(np.float32)
+    self.set_params(flat)
+    self.step_count = 0
+
+  def _offsets(self):
+    D, H = self.D, self.H
+    o = {}
+    o['W1'] = 0
+    o['b1'] = o['W1'] + H * D
+    o['W2'] = o['b1'] + H
+    o['b2'] = o['W2'] + H * H
+    o['W3'] = o['b2'] + H
+    o['b3'] = o['W3'] + D * (H + D)
+    return o
+
+  def init_skip_least_squares(self, W, G, max_rows=32768, ridge=1e-6):
+    """Start the output layer's linear skip at the least-squares map from the
+    normalised inputs to the normalised targets (on a row sample, FP64)."""
+    P = int(W.shape[0])
+    idx = np.linspace(0, P - 1, min(P, max_rows)).astype(np.int64)
+    sel = torch.as_tensor(idx, device=self.engine.device)
+    w = W.index_select(0, sel).double().cpu().numpy()
+    g = G.index_select(0, sel).double().cpu().numpy()
+    ins, insc, outs, outsc = [v.double().cpu().numpy() for v in self._norm]
+    x = (w - ins) * insc
+    t = (g - outs) / outsc
+    A = np.linalg.solve(x.T @ x + ridge * len(x) * np.eye(self.D), x.T @ t)   # t ~ x A
+    flat = self.get_params().cpu().numpy()
+    o = self._offsets()
+    W3 = flat[o['W3']:o['b3']].reshape(self.D, self.H + self.D)
+    W3[:, self.H:] = A.T
+    W3[:, :self.H] = 0.0
+    flat[o['b3']:o['b3'] + self.D] = 0.0
+    self.set_params(flat)
+    self.step_count = 0
+
+  def set_params(self, flat):
+    p = torch.as_tensor(np.asarray(flat, np.float32)).to(self.engine.device)
+    if p.numel() != self.n_params:
+      raise ValueError('expected %d parameters, got %d' % (self.n_params, p.numel()))
+    self.engine.check(self.engine.lib.edhmc_sur_set_params(self.engine.h, L.ptr(p),
+                                                           self.engine.stream))
+    torch.cuda.synchronize(self.engine.device)
+
+  def get_params(self):
+    p = torch.empty(self.n_params, device=self.engine.device)
+    self.engine.check(self.engine.lib.edhmc_sur_get_params(self.engine.h, L.ptr(p),
+                                                           self.engine.stream))
+    return p
+
+  def set_normalisation(self, in_shift, in_scale, out_shift, out_scale):
+    dev = self.engine.device
+    vs = [torch.as_tensor(v, dtype=torch.float32).to(dev).contiguous().reshape(-1)
+          for v in (in_shift, in_scale, out_shift, out_scale)]
+    self._norm = vs
+    self.engine.check(self.engine.lib.edhmc_sur_set_norm(
+        self.engine.h, *[L.ptr(v) for v in vs], self.engine.stream))
+
+  def normalise_from(self, W, G):
+    """Per-coordinate standardisation of inputs and targets from the data."""
+    wm, ws = self.engine.col_stats(W)
+    gm, gs = self.engine.col_stats(G)
+    ws = torch.where(ws > 0, ws, torch.ones_like(ws))
+    gs = torch.where(gs > 0, gs, torch.ones_like(gs))
+    self.set_normalisation(wm, 1.0 / ws, gm, gs)
+
+  # -- compute -------------------------------------------------------------
+  def forward(self, W):
+    W = W.to(self.engine.device, torch.float32).contiguous()
+    out = torch.empty_like(W)
+    for s in range(0, int(W.shape[0]), self.max_batch):
+      Wc = W[s:s + self.max_batch]
+      oc = out[s:s + self.max_batch]
+      self.engine.check(self.engine.lib.edhmc_sur_forward(
+          self.engine.h, L.ptr(Wc), int(Wc.shape[0]), L.ptr(oc), self.engine.stream))
+    return out
+
+  def loss_grad(self, W, G):
+    W = W.to(self.engine.device, torch.float32).contiguous()
+    G = G.to(self.engine.device, torch.float32).contiguous()
+    loss = torch.empty(1, dtype=torch.float64, device=self.engine.device)
+    grad = torch.empty(self.n_params, device=self.engine.device)
+    self.engine.check(self.engine.lib.edhmc_sur_loss_grad(
+        self.engine.h, L.ptr(W), L.ptr(G), int(W.shape[0]), L.ptr(loss), L.ptr(grad),
+        self.engine.stream))
+    return loss, grad
+
+  def train_step(self, W, G, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+    self.step_count += 1
+    loss = torch.empty(1, dtype=torch.float64, device=self.engine.device)
+    self.engine.check(self.engine.lib.edhmc_sur_train_step(
+        self.engine.h, L.ptr(W), L.ptr(G), int(W.shape[0]), float(lr), float(beta1),
+        float(beta2), float(eps), self.step_count, L.ptr(loss), self.engine.stream))
+    return loss
+
+  def fit(self, W, G, epochs=20, batch=4096, lr=1e-3, lr_final=1e-4, seed=0, normalise=True,
+          log=None):
+    """Adam on the pairs (W [P, D], G [P, D], device tensors), shuffled every
+    epoch, learning rate decayed geometrically from lr to lr_final.
+    Returns the mean loss of each epoch."""
+    W = W.to(self.engine.device, torch.float32).contiguous()
+    G = G.to(self.engine.device, torch.float32).contiguous()
+    P = int(W.shape[0])
+    batch = min(int(batch), self.max_batch, P)
+    if normalise:
+      self.normalise_from(W, G)
+      self.init_skip_least_squares(W, G)
+    gen = torch.Generator(device=self.engine.device)
+    gen.manual_seed(int(seed))
+    n_batches = P // batch
+    total = max(1, epochs * n_batches)
+    decay = (lr_final / lr) ** (1.0 / total) if lr_final else 1.0
+    losses = []
+    step = 0
+    for ep in range(epochs):
+      perm = torch.randperm(P, generator=gen, device=self.engine.device)
+      acc = []
+      for b in range(n_batches):
+        idx = perm[b * batch:(b + 1) * batch]
+        Wb = W.index_select(0, idx).contiguous()
+        Gb = G.index_select(0, idx).contiguous()
+        acc.append(self.train_step(Wb, Gb, lr=lr * decay ** step))
+        step += 1
+      ep_loss = torch.cat(acc).mean().item()
+      losses.append(ep_loss)
+      if log:
+        log('surrogate epoch %d loss %.5f' % (ep, ep_loss))
+    return losses
+
+  # -- sampling ------------------------------------------------------------
+  def attach(self, engine):
+    if engine is not self.engine:
+      raise ValueError('the surrogate belongs to another engine')
+
+  def transition(self, step_size, n_steps, momenta=None, uniforms=None, seed=0, counter=0,
+                 sample_out=None, n_accept=None, accepted_out=None):
+    eng = self.engine
+    r0 = None if momenta is None else torch.as_tensor(momenta, dtype=torch.float32).to(
+        eng.device).contiguous()
+    u = None if uniforms is None else torch.as_tensor(uniforms, dtype=torch.float64).to(
+        eng.device).contiguous()
+    eng.check(eng.lib.edhmc_transition_surrogate(
+        eng.h, float(step_size), int(n_steps), L.ptr(r0), L.ptr(u), int(seed), int(counter),
+        L.ptr(accepted_out), L.ptr(sample_out), L.ptr(n_accept), eng.stream))

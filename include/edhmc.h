@@ -140,6 +140,27 @@ int edhmc_moments_get(edhmc_ctx* ctx, double* mean_out, double* m2_out,
 int edhmc_profile_enable(edhmc_ctx* ctx, int32_t enable);
 int edhmc_profile_read(edhmc_ctx* ctx, double* total_ms_out, int64_t* count_out);
 
+/* Record the (position, exact gradient) pair of every inner leapfrog point of
+ * the following plain transitions into W_buf / G_buf [capacity, D] (rows
+ * appended chain-major per leapfrog step) until full; W_buf = NULL stops.
+ * These pairs train the gradient surrogate. */
+int edhmc_record_pairs(edhmc_ctx* ctx, float* W_buf, float* G_buf, int64_t capacity);
+/* Rows recorded by the current (or the last stopped) recording. */
+int64_t edhmc_record_count(edhmc_ctx* ctx);
+
+/* Empirical.mean / Empirical.variance (edward/models/empirical.py:87-96) of a
+ * sample store params [T, K] (K = product of the event shape) over rows
+ * t0 .. T-1: two-pass FP64, divisor T - t0 (population variance, as the
+ * reference's stddev).  params_dtype EDHMC_F32 or 2 (= fp64);
+ * mean_out, var_out [K] fp64. */
+int edhmc_sample_moments(edhmc_ctx* ctx, const void* params, int32_t params_dtype, int64_t T,
+                         int64_t K, int64_t t0, double* mean_out, double* var_out, void* stream);
+
+/* Column means and standard deviations (FP64 accumulation, divisor P) of
+ * X [P, D]: mean_out, std_out [D] fp32.  Deterministic. */
+int edhmc_col_stats(edhmc_ctx* ctx, const float* X, int64_t P, int32_t D, float* mean_out,
+                    float* std_out, void* stream);
+
 /* Number of chains whose proposal was non-finite (rejected) since binding,
  * [C] int32 device array copy. */
 int edhmc_divergences(edhmc_ctx* ctx, int32_t* out, void* stream);
@@ -162,12 +183,15 @@ int edhmc_autocov(edhmc_ctx* ctx, const float* s, int64_t T, int32_t C, int32_t 
 
 /* ------------------------------------------------------------------------
  * Learned gradient surrogate (no reference counterpart; build-only).
- * An MLP D -> H -> H -> D with tanh-GELU, fp32 master weights, bf16 MFMA
- * GEMMs with fp32 accumulation, trained with Adam on (w, grad log p) pairs.
- * It drives the inner leapfrog steps of edhmc_transition_surrogate; the exact
- * log joint still decides every Metropolis test.
+ * An MLP D -> H -> H -> D with tanh-GELU whose output layer also reads the
+ * normalised input (linear skip): out = W3 [GELU(a2) | x] + b3.  fp32 master
+ * weights, bf16 MFMA GEMMs with fp32 accumulation, trained with Adam on
+ * (w, grad log p) pairs.  It drives the inner leapfrog steps of
+ * edhmc_transition_surrogate; the exact log joint still decides every
+ * Metropolis test.
  * Weight layout (fp32, one flat buffer, PyTorch Linear convention):
- *   W1 [H, D] b1 [H] W2 [H, H] b2 [H] W3 [D, H] b3 [D]
+ *   W1 [H, D] b1 [H] W2 [H, H] b2 [H] W3 [D, H + D] b3 [D]
+ * (W3's first H columns act on the hidden layer, the last D on x).
  * Normalisation: x = (w - in_shift) * in_scale,  grad = out * out_scale + out_shift.
  * ------------------------------------------------------------------------ */
 int edhmc_sur_create(edhmc_ctx* ctx, int32_t D, int32_t H, int32_t max_batch);

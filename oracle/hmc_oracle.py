@@ -438,7 +438,7 @@ def autocovariance(x):
   return ac / n
 
 
-def ess(x):
+def ess(x, between=True):
   """Multi-chain effective sample size of draws x [M chains, n draws].
 
   Gelman et al. (BDA3, sec. 11.5): rho_t = 1 - (W - mean_m acov_m(t)) / var+,
@@ -446,19 +446,23 @@ def ess(x):
   of autocorrelations is truncated by Geyer's (1992) initial positive
   sequence of pair sums rho_{2k} + rho_{2k+1}, made monotone, and
   tau = -1 + 2 sum_k P_k, floored at 1/log10(M n).  ESS = M n / tau.
+  With between=False the between-chain term B/n is left out, so rho_t is the
+  pooled within-chain autocorrelation and chains that never move give 0.
   """
   x = np.atleast_2d(np.asarray(x, np.float64))
   M, n = x.shape
   if n < 4:
     return float('nan')
   acov = autocovariance(x)
-  chain_var = acov[:, 0] * n / (n - 1.0)
+  chain_var = x.var(axis=1, ddof=1)     # exact (the FFT lag 0 carries rounding)
+  chain_var[np.ptp(x, axis=1) == 0] = 0.0   # a constant chain has no variance
+  acov[:, 0] = chain_var * (n - 1.0) / n
   W = chain_var.mean()
   var_plus = W * (n - 1.0) / n
-  if M > 1:
+  if between and M > 1:
     var_plus += x.mean(axis=1).var(ddof=1)
   if not var_plus > 0:
-    return float('nan')
+    return 0.0 if not between else float('nan')
   rho = 1.0 - (W - acov.mean(axis=0)) / var_plus
   rho[0] = 1.0
   # Geyer initial positive sequence over pairs P_k = rho_2k + rho_2k+1

@@ -1,0 +1,95 @@
+"""FP64 NumPy restatement of the gradient-surrogate MLP -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/`` may import this module; it checks the HIP forward, backward
+and Adam kernels of ``libedhmc.so``.  The surrogate has no counterpart in the
+reference (Edward differentiates the log joint, ``edward/inferences/hmc.py:199``),
+so it restates published methods:
+
+  * tanh-GELU, GELU(x) = x/2 (1 + tanh(sqrt(2/pi)(x + 0.044715 x^3)))
+    (Hendrycks & Gimpel 2016), and its derivative;
+  * MLP D -> H -> H -> D with PyTorch's Linear weight layout [out, in], whose
+    output layer also reads the normalised input (linear skip):
+    out = W3 [GELU(a2) | x] + b3 with W3 [D, H + D];
+  * loss = mean over batch and coordinates of (out - (g - out_shift)/out_scale)^2
+    with inputs x = (w - in_shift) * in_scale;
+  * Adam (Kingma & Ba 2015) with bias correction.
+
+Pinning: the backward pass is checked against central finite differences of
+the forward loss (tests/test_mlp_oracle.py), GELU' against a finite difference
+of GELU, and Adam against its first two steps written out by hand.
+"""
+
+import numpy as np
+
+K0 = np.sqrt(2.0 / np.pi)
+K1 = 0.044715
+
+
+def gelu(x):
+  return 0.5 * x * (1.0 + np.tanh(K0 * (x + K1 * x ** 3)))
+
+
+def gelu_grad(x):
+  t = np.tanh(K0 * (x + K1 * x ** 3))
+  return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * K0 * (1.0 + 3.0 * K1 * x * x)
+
+
+def unflatten(p, D, H):
+  p = np.asarray(p, np.float64)
+  shapes = [('W1', (H, D)), ('b1', (H,)), ('W2', (H, H)), ('b2', (H,)), ('W3', (D, H + D)),
+            ('b3', (D,))]
+  out, o = {}, 0
+  for k, s in shapes:
+    n = int(np.prod(s))
+    out[k] = p[o:o + n].reshape(s)
+    o += n
+  assert o == p.size
+  return out
+
+
+def flatten(d):
+  return np.concatenate([d[k].ravel() for k in ('W1', 'b1', 'W2', 'b2', 'W3', 'b3')])
+
+
+def forward(params, W, norm):
+  x = (np.asarray(W, np.float64) - norm['in_shift']) * norm['in_scale']
+  a1 = x @ params['W1'].T + params['b1']
+  h1 = gelu(a1)
+  a2 = h1 @ params['W2'].T + params['b2']
+  h2 = gelu(a2)
+  cat = np.concatenate([h2, x], axis=1)
+  o = cat @ params['W3'].T + params['b3']
+  return dict(x=x, a1=a1, h1=h1, a2=a2, h2=h2, cat=cat, o=o)
+
+
+def predict(params, W, norm):
+  """Surrogate gradient = out * out_scale + out_shift."""
+  return forward(params, W, norm)['o'] * norm['out_scale'] + norm['out_shift']
+
+
+def loss_and_grad(params, W, G, norm):
+  f = forward(params, W, norm)
+  t = (np.asarray(G, np.float64) - norm['out_shift']) / norm['out_scale']
+  B, D = t.shape
+  d = f['o'] - t
+  loss = (d ** 2).sum() / (B * D)
+  dO = 2.0 * d / (B * D)
+  g = {}
+  g['W3'] = dO.T @ f['cat']
+  g['b3'] = dO.sum(0)
+  H = f['h2'].shape[1]
+  da2 = (dO @ params['W3'][:, :H]) * gelu_grad(f['a2'])
+  g['W2'] = da2.T @ f['h1']
+  g['b2'] = da2.sum(0)
+  da1 = (da2 @ params['W2']) * gelu_grad(f['a1'])
+  g['W1'] = da1.T @ f['x']
+  g['b1'] = da1.sum(0)
+  return loss, g
+
+
+def adam(p, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
+  m = b1 * m + (1 - b1) * g
+  v = b2 * v + (1 - b2) * g * g
+  mh = m / (1 - b1 ** t)
+  vh = v / (1 - b2 ** t)
+  return p - lr * mh / (np.sqrt(vh) + eps), m, v

@@ -1,0 +1,206 @@
+"""GPU checks of the gradient surrogate (libedhmc.so MLP kernels) against the
+FP64 MLP oracle, and of surrogate-driven HMC against the exact posterior.
+
+Bars: the GPU computes with bf16 weights/activations on the matrix cores and
+fp32 accumulation, so loss and parameter gradients are compared to the FP64
+oracle at 3e-2 relative (norm-wise), the forward pass at 2e-2; Adam on the
+same gradient to 1e-5.  Surrogate HMC must reproduce the exact posterior
+moments within Monte Carlo error even with an untrained network (the exact
+log joint decides acceptance)."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import hmc_oracle as O
+from oracle import mlp_oracle as MO
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(N=2000, D=24, seed=0, C=64):
+  from edward_amd import workloads as W
+  from edward_amd.engine import GLMEngine
+  wl = W.logreg(N, D, C, 0.01, 10, 10, seed=seed, bf16=True, w_sd=0.5)
+  eng = GLMEngine(wl.X, wl.y, wl.prior_loc, wl.prior_scale, precision='bf16')
+  return wl, eng
+
+
+def _surrogate(eng, H=64, B=512, seed=0):
+  from edward_amd.surrogate import Surrogate
+  return Surrogate(eng, hidden=H, max_batch=B, seed=seed)
+
+
+def _norm_np(rng, D):
+  return {'in_shift': rng.normal(size=D) * 0.1, 'in_scale': rng.uniform(0.5, 2, size=D),
+          'out_shift': rng.normal(size=D), 'out_scale': rng.uniform(0.5, 3, size=D)}
+
+
+def _set_norm(sur, norm):
+  sur.set_normalisation(*[norm[k].astype(np.float32) for k in
+                          ('in_shift', 'in_scale', 'out_shift', 'out_scale')])
+
+
+def _rel(a, b):
+  return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
+
+
+@pytest.mark.parametrize('D,H,B', [(24, 64, 300), (40, 200, 129)])
+def test_loss_grad_and_forward_match_oracle(D, H, B):
+  _, eng = _engine(D=D)
+  sur = _surrogate(eng, H=H, B=512, seed=1)
+  rng = np.random.default_rng(2)
+  norm = _norm_np(rng, D)
+  _set_norm(sur, norm)
+  W = rng.normal(size=(B, D)).astype(np.float32)
+  G = rng.normal(size=(B, D)).astype(np.float32) * 3
+  loss, grad = sur.loss_grad(torch.tensor(W), torch.tensor(G))
+  p = MO.unflatten(sur.get_params().cpu().numpy(), D, H)
+  loss_ref, g_ref = MO.loss_and_grad(p, W.astype(np.float64), G.astype(np.float64), norm)
+  assert loss.item() == pytest.approx(loss_ref, rel=3e-2)
+  gg = MO.unflatten(grad.cpu().numpy(), D, H)
+  for k in ('W1', 'b1', 'W2', 'b2', 'W3', 'b3'):
+    assert _rel(gg[k], g_ref[k]) < 3e-2, k
+  pred = sur.forward(torch.tensor(W)).cpu().numpy()
+  assert _rel(pred, MO.predict(p, W.astype(np.float64), norm)) < 2e-2
+
+
+def test_adam_step_matches_oracle_on_the_same_gradient():
+  _, eng = _engine(D=24)
+  sur = _surrogate(eng, H=64, B=256, seed=3)
+  rng = np.random.default_rng(4)
+  _set_norm(sur, _norm_np(rng, 24))
+  W = torch.tensor(rng.normal(size=(256, 24)).astype(np.float32), device='cuda')
+  G = torch.tensor(rng.normal(size=(256, 24)).astype(np.float32), device='cuda')
+  p0 = sur.get_params().cpu().numpy().astype(np.float64)
+  _, grad = sur.loss_grad(W, G)
+  sur.train_step(W, G, lr=1e-2)
+  p1 = sur.get_params().cpu().numpy()
+  want, _, _ = MO.adam(p0, grad.cpu().numpy().astype(np.float64), 0.0, 0.0, 1, lr=1e-2)
+  np.testing.assert_allclose(p1, want, atol=1e-5, rtol=1e-5)
+
+
+def test_training_learns_a_nonlinear_gradient_field():
+  """Field -P (w - mu) + 2 tanh(3 (w - mu)): the least-squares start of the
+  linear skip captures the first term; the hidden layers must learn the
+  second.  The trained surrogate reproduces the field on fresh points to 3%."""
+  D = 32
+  _, eng = _engine(D=D)
+  sur = _surrogate(eng, H=256, B=1024, seed=5)
+  rng = np.random.default_rng(6)
+  A = rng.normal(size=(D, D)) / math.sqrt(D)
+  P = A @ A.T + np.eye(D)
+  mu = rng.normal(size=D)
+  cov = np.linalg.inv(P)
+
+  def field(w):
+    return -(w - mu) @ P + 2.0 * np.tanh(3.0 * (w - mu))
+  Wtr = rng.multivariate_normal(mu, cov, size=20000)
+  losses = sur.fit(torch.tensor(Wtr, dtype=torch.float32),
+                   torch.tensor(field(Wtr), dtype=torch.float32),
+                   epochs=40, batch=1024, lr=2e-3, lr_final=1e-4)
+  assert losses[-1] < 0.5 * losses[0]
+  Wte = rng.multivariate_normal(mu, cov, size=2000)
+  pred = sur.forward(torch.tensor(Wte, dtype=torch.float32)).cpu().numpy()
+  assert _rel(pred, field(Wte)) < 0.03
+
+
+def test_surrogate_trajectory_matches_oracle_leapfrog():
+  """Fed momenta, u = 0 (accept): the end point of the surrogate-driven
+  leapfrog equals the oracle's leapfrog under the oracle MLP with the same
+  weights, to the bf16 precision of the network (5e-3 relative)."""
+  D, C = 24, 64
+  wl, eng = _engine(D=D, C=C)
+  sur = _surrogate(eng, H=64, B=C, seed=7)
+  rng = np.random.default_rng(8)
+  norm = {'in_shift': np.zeros(D), 'in_scale': np.ones(D) * 2.0,
+          'out_shift': np.zeros(D), 'out_scale': np.ones(D) * 5.0}
+  _set_norm(sur, norm)
+  W0 = (wl.w_true + 0.05 * rng.normal(size=(C, D))).astype(np.float32)
+  r0 = rng.normal(size=(C, D)).astype(np.float32)
+  eng.set_state(W0)
+  acc = torch.zeros(C, dtype=torch.int32, device='cuda')
+  sur.transition(0.01, 6, momenta=r0, uniforms=np.zeros(C), accepted_out=acc)
+  z, _, _ = eng.get_state()
+  p = MO.unflatten(sur.get_params().cpu().numpy(), D, 64)
+
+  def field(w):
+    return None, MO.predict(p, w, norm)
+  zL, rL, _, _ = O.leapfrog(W0.astype(np.float64), r0.astype(np.float64), 0.01, 6, field)
+  err = np.linalg.norm(z.cpu().numpy() - zL, axis=1) / np.linalg.norm(zL, axis=1)
+  assert acc.cpu().numpy().all()
+  assert err.max() < 5e-3
+
+
+def _posterior_moments(params):
+  s = params.reshape(-1, params.shape[-1])
+  return s.mean(0), s.var(0)
+
+
+def test_untrained_surrogate_still_samples_the_exact_posterior():
+  """Exactness: with a random (untrained) network as the force field, the
+  Metropolis test on the exact log joint keeps the FP64 posterior (Laplace on
+  N=4000 rows; its O(1/N) bias is far below the tolerance)."""
+  D, C, T = 6, 512, 300
+  from edward_amd import workloads as Wk
+  from edward_amd.engine import GLMEngine
+  wl = Wk.logreg(4000, D, C, 0.01, 10, 10, seed=9, bf16=True, w_sd=0.5)
+  eng = GLMEngine(wl.X, wl.y, wl.prior_loc, wl.prior_scale, precision='bf16')
+  sur = _surrogate(eng, H=64, B=C, seed=10)
+  m = O.GLM(wl.X.astype(np.float64), wl.y, wl.prior_loc, wl.prior_scale)
+  w_map, cov = O.laplace_logreg(m)
+  sd = np.sqrt(np.diag(cov))
+  # a crude, untrained force: identity network scaled by the posterior size
+  _set_norm(sur, {'in_shift': w_map, 'in_scale': 1.0 / sd, 'out_shift': np.zeros(D),
+                  'out_scale': 1.0 / sd})
+  rng = np.random.default_rng(11)
+  eng.set_state((w_map + sd * rng.normal(size=(C, D))).astype(np.float32))
+  params = torch.zeros(T, C, D, device='cuda')
+  n_acc = torch.zeros(C, dtype=torch.int32, device='cuda')
+  for t in range(T):
+    sur.transition(0.3 * sd.min(), 4, seed=12, counter=t, sample_out=params[t], n_accept=n_acc)
+  acc_rate = n_acc.sum().item() / (C * T)
+  assert 0.01 < acc_rate < 0.99
+  s = params[T // 3:].cpu().numpy().astype(np.float64)
+  mean, var = _posterior_moments(s)
+  n_eff = s.shape[0] * C * max(acc_rate, 0.05) / 10   # conservative ESS
+  assert np.all(np.abs(mean - w_map) < 5 * sd / math.sqrt(n_eff))
+  np.testing.assert_allclose(var, sd ** 2, rtol=0.2)
+
+
+def test_trained_surrogate_gains_over_untrained():
+  """Acceptance of surrogate HMC with a network trained on recorded warm-up
+  pairs versus the same network untrained, at the same eps and L."""
+  D, C = 24, 256
+  wl, eng = _engine(N=4000, D=D, C=C, seed=13)
+  rng = np.random.default_rng(14)
+  m = O.GLM(wl.X.astype(np.float64), wl.y, wl.prior_loc, wl.prior_scale)
+  w_map, cov = O.laplace_logreg(m)
+  sd = np.sqrt(np.diag(cov))
+  eps, L = 0.5 * sd.min(), 10
+  eng.set_state((w_map + sd * rng.normal(size=(C, D))).astype(np.float32))
+  cap = C * (L - 1) * 8
+  Wb = torch.empty(cap, D, device='cuda')
+  Gb = torch.empty(cap, D, device='cuda')
+  for t in range(10):
+    if t == 2:
+      eng.record_pairs(Wb, Gb)
+    eng.transition(eps, L, seed=15, counter=t)
+  eng.record_pairs(None)
+  torch.cuda.synchronize()
+  sur = _surrogate(eng, H=128, B=C, seed=16)
+  untrained = sur.get_params().clone()
+
+  def acceptance():
+    n = torch.zeros(C, dtype=torch.int32, device='cuda')
+    for t in range(20):
+      sur.transition(eps, L, seed=17, counter=1000 + t, n_accept=n)
+    return n.sum().item() / (20.0 * C)
+  sur.normalise_from(Wb, Gb)
+  a_untrained = acceptance()
+  sur.set_params(untrained.cpu().numpy())
+  sur.fit(Wb, Gb, epochs=40, batch=min(1024, C * 4), lr=2e-3, lr_final=1e-4)
+  a_trained = acceptance()
+  assert a_trained > 0.6 and a_trained > a_untrained + 0.1, (a_untrained, a_trained)

@@ -317,6 +317,22 @@ def test_ess_ar1_closed_form(rho):
     assert got == pytest.approx(want, rel=0.1)
 
 
+@pytest.mark.parametrize('rho', [0.0, 0.7])
+def test_within_chain_ess_ar1_and_stuck_chains(rho):
+  rng = np.random.default_rng(16)
+  M, n = 16, 5000
+  x = np.zeros((M, n))
+  e = rng.normal(size=(M, n))
+  x[:, 0] = e[:, 0] / math.sqrt(1 - rho ** 2)
+  for t in range(1, n):
+    x[:, t] = rho * x[:, t - 1] + e[:, t]
+  assert O.ess(x, between=False) == pytest.approx(M * n * (1 - rho) / (1 + rho), rel=0.1)
+  # chains frozen at spread-out starting points: no sampling happened
+  frozen = np.repeat(rng.normal(size=(M, 1)), n, axis=1)
+  assert O.ess(frozen, between=False) == 0.0
+  assert O.ess(frozen, between=True) > 0     # the BDA3 form credits the spread
+
+
 def test_min_ess_takes_worst_coordinate():
   rng = np.random.default_rng(14)
   T, C = 2000, 4
