@@ -49,10 +49,19 @@ def parse():
   ap.add_argument('--steps', type=int, default=200, help='timed surrogate transitions')
   ap.add_argument('--warmup', type=int, default=40, help='max plain bf16 warm-up transitions')
   ap.add_argument('--plain-steps', type=int, default=8, help='timed plain fp32 transitions')
-  ap.add_argument('--workload', default='large', choices=['large', 'mid', 'small'])
+  ap.add_argument('--workload', default='large', choices=['large', 'mid', 'small', 'hier'],
+                  help="'hier': the hierarchical logistic regression (G=2048 groups of "
+                       "1+Poisson(400) rows, D=64, 512 chains, L=16), its own JSON line")
   ap.add_argument('--hidden', type=int, default=1024)
   ap.add_argument('--epochs', type=int, default=12)
+  ap.add_argument('--lr', type=float, default=3e-3)
+  ap.add_argument('--rec-stride', type=int, default=4)
+  ap.add_argument('--rec-transitions', type=int, default=8)
+  ap.add_argument('--surrogate-only', action='store_true',
+                  help='stop after the surrogate phase (tuning runs)')
   ap.add_argument('--seed', type=int, default=0)
+  ap.add_argument('--jitter', type=float, default=0.2,
+                  help='step size drawn from eps * U(1-j, 1+j) per transition (both samplers)')
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--cpu-seconds', type=float, default=15.0)
   return ap.parse_args()
@@ -102,6 +111,57 @@ def cpu_baseline(wl, budget_s, ess_per_transition_plain):
   }
 
 
+def bench_hier(args):
+  """Plain fp32 HMC on the hierarchical model: transitions/s and the
+  gradient GEMM's share, with the device profiler over the timed region."""
+  import torch
+  from edward_amd import workloads as W
+  from edward_amd.hier import HierEngine
+  wl = W.hier_groups(seed=args.seed)
+  log('hier workload G=%d N=%d D=%d C=%d L=%d' % (wl.G, wl.N, wl.D, wl.C, wl.n_steps))
+  eng = HierEngine(wl.X, wl.y, wl.offsets)
+  eng.set_state(wl.theta0)
+  C, L, eps = wl.C, wl.n_steps, wl.step_size
+  dev = torch.device('cuda', 0)
+  n_acc = torch.zeros(C, dtype=torch.int32, device=dev)
+  for t in range(args.warmup):
+    eng.transition(eps, L, seed=args.seed, counter=t, n_accept=n_acc)
+  torch.cuda.synchronize(dev)
+  n_acc.zero_()
+  K = args.steps
+  eng.profile(True)
+  t0 = time.time()
+  for t in range(K):
+    eng.transition(eps, L, seed=args.seed, counter=args.warmup + t, n_accept=n_acc)
+  torch.cuda.synchronize(dev)
+  secs = time.time() - t0
+  prof = eng.profile_read()
+  acc = float(n_acc.double().mean().item()) / K
+  # one gradient pass: eta = X w (2NDC) and X^T R (2NDC) flops
+  flops = 4.0 * wl.N * wl.D * C
+  g_ms, g_n = prof['gemm_grad']
+  x_ms, x_n = prof['gemm_xtr']
+  ll_ms, ll_n = prof['gemm_ll']
+  n_grad = g_n + ll_n
+  gemm_ms = (g_ms + ll_ms + x_ms) / max(n_grad, 1)
+  achieved = flops / (gemm_ms * 1e-3)
+  log('hier: %.2f ms/transition, accept %.3f, profile %s' % (secs / K * 1e3, acc, prof))
+  out = {
+      'metric': 'hierarchical logreg HMC transitions/sec (G=2048, D=64, 512 chains, L=16)',
+      'value': K / secs, 'unit': 'transitions/s', 'n_gpus': 1, 'steps': K,
+      'warmup': args.warmup, 'ms_per_step': secs / K * 1e3, 'higher_is_better': True,
+      'scaling': 'weak', 'vs_baseline': None, 'dtype': 'fp32', 'data': 'synthetic',
+      'config': {'workload': 'hier_logreg G=%d N=%d D=%d chains=%d eps=%g L=%d'
+                             % (wl.G, wl.N, wl.D, C, eps, L)},
+      'accept': acc,
+      'profile_ms_per_transition': {k: v[0] / K for k, v in prof.items()},
+      'roofline': {'bound': 'mfma', 'kernel': 'gemm_grad+gemm_xtr (one gradient pass)',
+                   'achieved': achieved / 1e12, 'peak': PEAK['fp32'] / 1e12,
+                   'unit': 'TFLOP/s', 'frac': achieved / PEAK['fp32'], 'traffic': None},
+  }
+  print(json.dumps(out))
+
+
 def main():
   args = parse()
   if args.gpus != 1:
@@ -109,10 +169,13 @@ def main():
           'multi-GPU scaling is not part of it (requested --gpus %d)' % args.gpus,
           file=sys.stderr)
     sys.exit(2)
+  if args.workload == 'hier':
+    return bench_hier(args)
   import torch
   from edward_amd import diagnostics
   from edward_amd.engine import GLMEngine
   from edward_amd.surrogate import Surrogate
+  from edward_amd.engine import jittered_step
 
   wl = make_workload(args.workload)
   N, D, C, eps, L = wl.N, wl.D, wl.C, wl.step_size, wl.n_steps
@@ -125,8 +188,12 @@ def main():
   # 1. warm-up with plain bf16 HMC; record pairs over the last two transitions
   t_warm0 = time.time()
   means = []
-  n_rec = 2
-  cap = (L - 1) * C * n_rec
+  # training pairs: every rec_stride-th inner leapfrog point of n_rec
+  # transitions (~250k pairs from more independent trajectories than a
+  # dense record of fewer transitions)
+  rec_stride = args.rec_stride
+  n_rec = args.rec_transitions
+  cap = ((L - 1) // rec_stride) * C * n_rec
   Wrec = torch.empty(cap, D, device=dev)
   Grec = torch.empty(cap, D, device=dev)
   n_acc = torch.zeros(C, dtype=torch.int32, device=dev)
@@ -149,9 +216,10 @@ def main():
     if converged_at is None and t >= 10 and abs(means[-1] - means[-6]) < 20.0:
       converged_at = t
       break
-  bf.record_pairs(Wrec, Grec)
+  bf.record_pairs(Wrec, Grec, stride=rec_stride)
   for k in range(n_rec):
-    bf.transition(eps, L, seed=args.seed, counter=t, n_accept=n_acc)
+    bf.transition(jittered_step(eps, args.jitter, args.seed, t), L, seed=args.seed, counter=t,
+                  n_accept=n_acc)
     t += 1
   bf.record_pairs(None)
   torch.cuda.synchronize()
@@ -162,8 +230,8 @@ def main():
   # 2. train the surrogate
   t_tr0 = time.time()
   sur = Surrogate(bf, hidden=args.hidden, max_batch=C, seed=args.seed)
-  losses = sur.fit(Wrec[:n_pairs], Grec[:n_pairs], epochs=args.epochs, batch=C, lr=1e-3,
-                   lr_final=5e-5, seed=args.seed, log=log)
+  losses = sur.fit(Wrec[:n_pairs], Grec[:n_pairs], epochs=args.epochs, batch=C, lr=args.lr,
+                   lr_final=5e-5, seed=args.seed, log=log, val_frac=0.05)
   torch.cuda.synchronize()
   train_s = time.time() - t_tr0
   del Wrec, Grec
@@ -174,13 +242,15 @@ def main():
   samples = torch.empty(K, C, D, device=dev)
   n_acc.zero_()
   for k in range(2):   # untimed: settle ghat and first-call overheads
-    sur.transition(eps, L, seed=args.seed, counter=t, n_accept=None)
+    sur.transition(jittered_step(eps, args.jitter, args.seed, t), L, seed=args.seed,
+                   counter=t, n_accept=None)
     t += 1
   bf.profile(True)
   torch.cuda.synchronize()
   ts0 = time.time()
   for k in range(K):
-    sur.transition(eps, L, seed=args.seed, counter=t, sample_out=samples[k], n_accept=n_acc)
+    sur.transition(jittered_step(eps, args.jitter, args.seed, t), L, seed=args.seed,
+                   counter=t, sample_out=samples[k], n_accept=n_acc)
     t += 1
   torch.cuda.synchronize()
   sur_s = time.time() - ts0
@@ -200,6 +270,11 @@ def main():
   del samples
   torch.cuda.empty_cache()
 
+  if args.surrogate_only:
+    print(json.dumps({'surrogate_min_ess_per_s': mess_sur / sur_s, 'accept': acc_sur,
+                      'final_loss': losses[-1], 'ms_per_step': sur_s / K * 1e3}))
+    return
+
   # 4. timed plain fp32 from the same equilibrium states
   f32 = GLMEngine(wl.X, wl.y, wl.prior_loc, wl.prior_scale, precision='fp32')
   f32.set_state(state)
@@ -208,7 +283,8 @@ def main():
   torch.cuda.synchronize()
   tp0 = time.time()
   for k in range(Kp):
-    f32.transition(eps, L, seed=args.seed + 1, counter=t + k, sample_out=psamples[k],
+    f32.transition(jittered_step(eps, args.jitter, args.seed + 1, t + k), L,
+                   seed=args.seed + 1, counter=t + k, sample_out=psamples[k],
                    n_accept=n_acc)
     torch.cuda.synchronize()
     log('plain fp32 transition %d' % k)
@@ -264,8 +340,9 @@ def main():
       'vs_baseline': None,
       'dtype': 'bf16',
       'data': 'synthetic',
-      'config': {'workload': 'logreg_hmc N=%d D=%d chains=%d eps=%g L=%d X=bf16 '
-                             'surrogate MLP %d-%d-%d-%d' % (N, D, C, eps, L, D, args.hidden,
+      'config': {'workload': 'logreg_hmc N=%d D=%d chains=%d eps=%g (x U(%.2f,%.2f) per '
+                             'transition) L=%d X=bf16 surrogate MLP %d-%d-%d-%d+skip'
+                             % (N, D, C, eps, 1 - args.jitter, 1 + args.jitter, L, D, args.hidden,
                                                             args.hidden, D),
                  'global_batch': C, 'seq_len': None, 'parallelism': 'none (1 GPU)'},
       'ess_estimator': 'multi-chain (Gelman et al. BDA3 11.5, Geyer initial monotone '

@@ -44,6 +44,7 @@ SIGNATURES = {
     'edhmc_set_state': (ctypes.c_int, [_c_p, _c_p, _i32, _c_p]),
     'edhmc_get_state': (ctypes.c_int, [_c_p, _c_p, _c_p, _c_p, _c_p]),
     'edhmc_logp_grad': (ctypes.c_int, [_c_p, _c_p, _i32, _c_p, _c_p, _c_p]),
+    'edhmc_logp': (ctypes.c_int, [_c_p, _c_p, _i32, _c_p, _c_p]),
     'edhmc_transition': (ctypes.c_int, [_c_p, _f32, _i32, _c_p, _c_p, _u64, _u64, _c_p,
                                         _c_p, _c_p, _c_p]),
     'edhmc_run': (ctypes.c_int, [_c_p, _f32, _i32, _c_p, _i64, _i64, _i64, _u64, _u64,
@@ -53,7 +54,7 @@ SIGNATURES = {
     'edhmc_divergences': (ctypes.c_int, [_c_p, _c_p, _c_p]),
     'edhmc_sample_moments': (ctypes.c_int, [_c_p, _c_p, _i32, _i64, _i64, _i64, _c_p, _c_p,
                                             _c_p]),
-    'edhmc_record_pairs': (ctypes.c_int, [_c_p, _c_p, _c_p, _i64]),
+    'edhmc_record_pairs': (ctypes.c_int, [_c_p, _c_p, _c_p, _i64, _i32]),
     'edhmc_record_count': (_i64, [_c_p]),
     'edhmc_col_stats': (ctypes.c_int, [_c_p, _c_p, _i64, _i32, _c_p, _c_p, _c_p]),
     'edhmc_profile_enable': (ctypes.c_int, [_c_p, _i32]),

@@ -24,7 +24,7 @@ enum ProfKind {
   PK_GEMM_XTR = 2,    // GEMM2 (X^T residual, split-K)
   PK_CHAIN = 3,       // per-chain leapfrog / accept kernels
   PK_MLP = 4,         // surrogate MLP GEMMs (+ fused epilogues)
-  PK_HIER = 5,        // hierarchical persistent kernel
+  PK_HIER = 5,        // hierarchical segmented reductions
   PK_COUNT = 6
 };
 
@@ -90,6 +90,7 @@ struct edhmc_ctx {
   float* rec_W = nullptr;
   float* rec_G = nullptr;
   int64_t rec_cap = 0, rec_count = 0;
+  int rec_stride = 1;
   // ---- misc scratch ----
   edhmc::DevBuf tmp_count, stats_part;
 

@@ -54,9 +54,29 @@ int prof_collect(edhmc_ctx* ctx) {
   return EDHMC_OK;
 }
 
-using Cfg1 = TileCfg<2, 2, 2, 2>;   // 128 x 128, GEMM1 and wide GEMM2
-using Cfg2S = TileCfg<1, 1, 4, 1>;  // 128 x 32, GEMM2 for D_pad % 128 != 0
-using CfgP2 = TileCfg<1, 2, 2, 2>;  // 64 x 128, GEMM1 over hi/lo split positions
+// Tile shapes (blocks per wave M, N; waves M, N).  The defaults can be
+// overridden at compile time to sweep alternatives.
+#ifndef EDHMC_CFG_G1
+#define EDHMC_CFG_G1 2, 2, 2, 2
+#endif
+#ifndef EDHMC_CFG_G2
+#define EDHMC_CFG_G2 4, 4, 2, 2
+#endif
+#ifndef EDHMC_CFG_P2
+#define EDHMC_CFG_P2 1, 4, 2, 2
+#endif
+using Cfg1 = TileCfg<EDHMC_CFG_G1>;    // GEMM1 (gradient pass, one plane)
+using CfgG2 = TileCfg<EDHMC_CFG_G2>;   // GEMM2 when D_pad % CfgG2::BN == 0
+using Cfg2S = TileCfg<1, 1, 4, 1>;     // 128 x 32, GEMM2 for narrow D
+using CfgP2 = TileCfg<EDHMC_CFG_P2>;   // GEMM1 over hi/lo split positions
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+// chains are padded to a multiple of every M tile (and of the 128-row MLP
+// batch tile); data rows to a multiple of every row tile
+constexpr int kChainPad = cmax(cmax(Cfg1::BM, CfgG2::BM), cmax(CfgP2::BM, 128));
+constexpr int kRowPad = cmax(cmax(Cfg1::BN, CfgP2::BN), 128);
+static_assert(kChainPad % Cfg1::BM == 0 && kChainPad % CfgG2::BM == 0 &&
+              kChainPad % CfgP2::BM == 0 && kChainPad % 128 == 0, "chain padding");
+static_assert(kRowPad % Cfg1::BN == 0 && kRowPad % CfgP2::BN == 0, "row padding");
 constexpr int kChainThreads = 256;
 
 // ---------------------------------------------------------------------------
@@ -72,31 +92,49 @@ struct EpiGLM {
   int64_t ldr;        // C_pad
   double* llpart;     // [n_tiles][C_pad]
   int64_t ll_ld;      // C_pad
+  // Optional per-row offset added to eta: goff[m * goff_ld + gidx[n]] (the
+  // group intercept a_{g(n)} of the hierarchical model); null for a GLM.
+  const float* goff = nullptr;
+  int64_t goff_ld = 0;
+  const int* gidx = nullptr;
 
+  // The residual tile goes through LDS when it fits beside the ll scratch:
+  // lanes write it row-major ([m][n], padded rows), then each thread stores
+  // whole 16-byte pieces of the K-blocked layout, in which the BM rows of
+  // one 64-byte K block are contiguous -- instead of 2-byte scattered stores.
   template <typename Cfg>
   __device__ void apply(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p) const {
     constexpr int WM = Cfg::kWM, WN = Cfg::kWN;
-    constexpr int BM = Cfg::BM;
+    constexpr int BM = Cfg::BM, BN = Cfg::BN, NT = Cfg::kThreads;
     constexpr int KE = kStageBytes / sizeof(T);
-    double* sm = reinterpret_cast<double*>(p.smem);
+    constexpr int RROW = BN * (int)sizeof(T) + 16;          // LDS row stride (bytes)
+    constexpr int RTILE = (BM * RROW + 15) / 16 * 16;
+    constexpr int LLB = Cfg::kWavesN * BM * 8;
+    const bool stage = WR && (RTILE + LLB <= p.smem_bytes);
+    char* rt = p.smem;
+    double* sm = reinterpret_cast<double*>(p.smem + (stage ? RTILE : 0));
     float yv[WN];
     bool valid[WN];
+    int gj[WN];
 #pragma unroll
     for (int j = 0; j < WN; ++j) {
       const int64_t n = p.n0 + 32 * j + (p.lane & 31);
       valid[j] = n < N;
       yv[j] = valid[j] ? y[n] : 0.0f;
+      gj[j] = (goff && valid[j]) ? gidx[n] : 0;
     }
+    const int ml0 = p.m0 - p.tm * BM, nl0 = p.n0 - p.tn * BN;   // tile-local origin
 #pragma unroll
     for (int i = 0; i < WM; ++i)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int64_t m = p.m0 + 32 * i + acc_row(r, p.lane);
+        const int mr = 32 * i + acc_row(r, p.lane);
         double llsum = 0.0;
+        const float* grow = goff ? goff + (int64_t)(p.m0 + mr) * goff_ld : nullptr;
 #pragma unroll
         for (int j = 0; j < WN; ++j) {
-          const int64_t n = p.n0 + 32 * j + (p.lane & 31);
-          const float eta = acc[i][j][r];
+          const int nc = 32 * j + (p.lane & 31);
+          const float eta = acc[i][j][r] + (grow ? grow[gj[j]] : 0.0f);
           float dll;
           if (LL) {
             double ll;
@@ -105,20 +143,35 @@ struct EpiGLM {
           } else {
             dll = glm_dll(lp, eta, yv[j]);
           }
-          if (WR)
-            Rt[((n / KE) * ldr + m) * KE + (n % KE)] =
-                ElemOps<T>::from_f32(valid[j] ? dll : 0.0f);
+          if (WR) {
+            const T v = ElemOps<T>::from_f32(valid[j] ? dll : 0.0f);
+            if (stage) {
+              *(T*)(rt + (ml0 + mr) * RROW + (nl0 + nc) * (int)sizeof(T)) = v;
+            } else {
+              const int64_t m = p.m0 + mr, n = p.n0 + nc;
+              Rt[((n / KE) * ldr + m) * KE + (n % KE)] = v;
+            }
+          }
         }
         if (LL) {
 #pragma unroll
           for (int off = 16; off > 0; off >>= 1) llsum += __shfl_xor(llsum, off, 64);
-          if ((p.lane & 31) == 0)
-            sm[p.wn * BM + p.wm * 32 * WM + 32 * i + acc_row(r, p.lane)] = llsum;
+          if ((p.lane & 31) == 0) sm[p.wn * BM + ml0 + mr] = llsum;
         }
       }
+    if (stage || LL) __syncthreads();
+    if (stage) {
+      // BN/KE K-blocks x BM rows x 4 pieces of 16 bytes
+      constexpr int PIECES = (BN / KE) * BM * 4;
+      char* base = (char*)Rt + ((int64_t)(p.tn * BN / KE) * ldr + (int64_t)p.tm * BM) * kStageBytes;
+      for (int id = threadIdx.x; id < PIECES; id += NT) {
+        const int part = id & 3, m = (id >> 2) % BM, nb = (id >> 2) / BM;
+        const uint4 v = *(const uint4*)(rt + m * RROW + nb * kStageBytes + part * 16);
+        *(uint4*)(base + ((int64_t)nb * ldr + m) * kStageBytes + part * 16) = v;
+      }
+    }
     if (LL) {
-      __syncthreads();
-      for (int t = threadIdx.x; t < BM; t += Cfg::kThreads) {
+      for (int t = threadIdx.x; t < BM; t += NT) {
         double s = 0.0;
 #pragma unroll
         for (int w = 0; w < Cfg::kWavesN; ++w) s += sm[w * BM + t];
@@ -407,6 +460,29 @@ __global__ void __launch_bounds__(kChainThreads) chain_load_kernel(ChainArgs a, 
   if (bad) atomicAdd(nonfinite, 1);
 }
 
+// Positions W [C, D] into the z_lp planes only (forward-only evaluation).
+__global__ void __launch_bounds__(kChainThreads) lp_load_kernel(ChainArgs a, int* nonfinite) {
+  const int c = blockIdx.x;
+  int bad = 0;
+  for (int d = threadIdx.x; d < a.D; d += kChainThreads) {
+    const float v = a.W_in[(int64_t)c * a.D + d];
+    if (!isfinite(v)) bad = 1;
+    write_lp(a, c, d, v);
+  }
+  if (bad) atomicAdd(nonfinite, 1);
+}
+
+// logp_out[c] = ll[c] + prior log density of W[c].
+__global__ void __launch_bounds__(kChainThreads) logp_out_kernel(ChainArgs a) {
+  __shared__ double red[kChainThreads / 64];
+  const int c = blockIdx.x;
+  double plp = 0.0;
+  for (int d = threadIdx.x; d < a.D; d += kChainThreads)
+    plp += prior_logp(a, d, a.W_in[(int64_t)c * a.D + d]);
+  plp = block_sum_f64<kChainThreads>(plp, red);
+  if (threadIdx.x == 0) a.logp_out[c] = a.ll[c] + plp;
+}
+
 // Rewrite z_lp from z for every chain.
 __global__ void __launch_bounds__(kChainThreads) chain_lp_kernel(ChainArgs a) {
   const int c = blockIdx.x;
@@ -570,7 +646,8 @@ static ChainArgs chain_args(edhmc_ctx* ctx) {
 }
 
 template <typename T, typename Cfg, int PLANES, bool WR, bool LL>
-static int launch_gemm1(edhmc_ctx* ctx, hipStream_t st) {
+static int launch_gemm1(edhmc_ctx* ctx, hipStream_t st, const float* goff = nullptr,
+                        int64_t goff_ld = 0, const int* gidx = nullptr) {
   const int eb = sizeof(T);
   GemmShape g = {};
   g.A = (const char*)ctx->z_lp.p;
@@ -595,12 +672,15 @@ static int launch_gemm1(edhmc_ctx* ctx, hipStream_t st) {
   e.ldr = ctx->C_pad;
   e.llpart = (double*)ctx->llpart.p;
   e.ll_ld = ctx->C_pad;
+  e.goff = goff;
+  e.goff_ld = goff_ld;
+  e.gidx = gidx;
   hipLaunchKernelGGL((gemm_nt_kernel<T, Cfg, PLANES, EpiGLM<T, WR, LL>>), dim3(grid_size(g)),
                      dim3(Cfg::kThreads), 0, st, g, e);
   EDHMC_TRY(check_launch(ctx, "gemm1"));
   if (LL) {
     hipLaunchKernelGGL(ll_reduce_kernel, dim3(ctx->C_pad / 64), dim3(256), 0, st,
-                       (const double*)ctx->llpart.p, ctx->n_tiles1, ctx->C_pad, (double*)ctx->ll.p);
+                       (const double*)ctx->llpart.p, g.n_tiles, ctx->C_pad, (double*)ctx->ll.p);
     EDHMC_TRY(check_launch(ctx, "ll_reduce"));
   }
   return EDHMC_OK;
@@ -635,7 +715,8 @@ static int launch_gemm2_cfg(edhmc_ctx* ctx, hipStream_t st) {
 
 template <typename T>
 static int launch_gemm2(edhmc_ctx* ctx, hipStream_t st) {
-  if (ctx->D_pad % Cfg1::BN == 0) return launch_gemm2_cfg<T, Cfg1>(ctx, st);
+  if (ctx->D_pad % CfgG2::BN == 0 && ctx->C_pad % CfgG2::BM == 0)
+    return launch_gemm2_cfg<T, CfgG2>(ctx, st);
   return launch_gemm2_cfg<T, Cfg2S>(ctx, st);
 }
 
@@ -657,6 +738,19 @@ static int grad_eval(edhmc_ctx* ctx, bool split, bool ll, hipStream_t st) {
   return launch_gemm2<float>(ctx, st);
 }
 
+// fp32 gradient pass with a per-row gathered offset in eta (hierarchical
+// model): residuals R^T, optional ll, then X^T R into the split-K partials.
+int glm_grad_eval_offset(edhmc_ctx* ctx, const float* goff, int64_t goff_ld, const int* gidx,
+                         bool ll, hipStream_t st) {
+  if (ctx->cdt != EDHMC_F32) return set_err(ctx, EDHMC_EUNSUPPORTED, "offset path is fp32");
+  if (ll) EDHMC_TRY((launch_gemm1<float, Cfg1, 1, true, true>(ctx, st, goff, goff_ld, gidx)));
+  else EDHMC_TRY((launch_gemm1<float, Cfg1, 1, true, false>(ctx, st, goff, goff_ld, gidx)));
+  return launch_gemm2<float>(ctx, st);
+}
+
+// Buffers the chain-level engines above the GLM need to size their state.
+int glm_ensure_chains(edhmc_ctx* ctx, int C);
+
 // Forward only (log-likelihood partials, no residuals) at z_lp, exact form.
 int glm_forward_ll(edhmc_ctx* ctx, hipStream_t st) {
   if (ctx->cdt == EDHMC_BF16) return launch_gemm1<bf16_t, CfgP2, 2, false, true>(ctx, st);
@@ -672,7 +766,7 @@ int glm_write_lp(edhmc_ctx* ctx, hipStream_t st) {
 static int ensure_chains(edhmc_ctx* ctx, int C) {
   if (C <= 0) return set_err(ctx, EDHMC_EINVAL, "number of chains must be positive, got %d", C);
   if (ctx->C == C && ctx->z.p) return EDHMC_OK;
-  const int C_pad = (int)round_up(C, Cfg1::BM);
+  const int C_pad = (int)round_up(C, kChainPad);
   const size_t cd = (size_t)C * ctx->D_pad;
   const int eb = elem_bytes(ctx);
   ctx->C = C;
@@ -701,11 +795,13 @@ static int ensure_chains(edhmc_ctx* ctx, int C) {
   if (ks >= 8) ks = ks / 8 * 8;
   ctx->ksplit = ks;
   EDHMC_TRY(dev_alloc(ctx, ctx->Gpart, (size_t)ks * C_pad * ctx->D_pad * 4, false));
-  ctx->n_tiles1 = (int)(ctx->N_pad / Cfg1::BN);
+  ctx->n_tiles1 = (int)(ctx->N_pad / std::min(Cfg1::BN, CfgP2::BN));   // llpart rows
   EDHMC_TRY(dev_alloc(ctx, ctx->llpart, (size_t)C_pad * ctx->n_tiles1 * 8, false));
   EDHMC_TRY(dev_alloc(ctx, ctx->ll, (size_t)C_pad * 8));
   return EDHMC_OK;
 }
+
+int glm_ensure_chains(edhmc_ctx* ctx, int C) { return ensure_chains(ctx, C); }
 
 static int do_set_state(edhmc_ctx* ctx, const float* W, int C, hipStream_t st) {
   if (!ctx->bound) return set_err(ctx, EDHMC_ESTATE, "no model bound");
@@ -769,7 +865,7 @@ static int do_transition(edhmc_ctx* ctx, float eps, int L, const float* r0, cons
     EDHMC_TRY(grad_eval(ctx, last, last, st));
     if (!last) {
       a.rec_W = a.rec_G = nullptr;
-      if (ctx->rec_W && ctx->rec_count + ctx->C <= ctx->rec_cap) {
+      if (ctx->rec_W && k % ctx->rec_stride == 0 && ctx->rec_count + ctx->C <= ctx->rec_cap) {
         a.rec_W = ctx->rec_W + ctx->rec_count * ctx->D;
         a.rec_G = ctx->rec_G + ctx->rec_count * ctx->D;
         ctx->rec_count += ctx->C;
@@ -864,7 +960,7 @@ int edhmc_bind_glm(edhmc_ctx* ctx, const void* X, int32_t x_in_dtype, const floa
   ctx->lik = likelihood;
   ctx->lik_scale = lik_scale;
   ctx->N = N;
-  ctx->N_pad = round_up(N, Cfg1::BN);
+  ctx->N_pad = round_up(N, kRowPad);
   ctx->D = D;
   ctx->D_pad = (int)round_up(D, 32);
   const int eb = elem_bytes(ctx);
@@ -948,6 +1044,30 @@ int edhmc_logp_grad(edhmc_ctx* ctx, const float* W, int32_t C, float* grad_out, 
   DeviceGuard dg(ctx->device);
   EDHMC_TRY(do_set_state(ctx, W, C, (hipStream_t)stream));
   return edhmc_get_state(ctx, nullptr, logp_out, grad_out, stream);
+}
+
+int edhmc_logp(edhmc_ctx* ctx, const float* W, int32_t C, double* logp_out, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  if (!ctx->bound) return set_err(ctx, EDHMC_ESTATE, "no model bound");
+  if (!W || !logp_out) return set_err(ctx, EDHMC_EINVAL, "NULL argument");
+  EDHMC_TRY(ensure_chains(ctx, C));
+  EDHMC_TRY(dev_alloc(ctx, ctx->tmp_count, 4));
+  ChainArgs a = chain_args(ctx);
+  a.W_in = W;
+  a.logp_out = logp_out;
+  hipLaunchKernelGGL(lp_load_kernel, dim3(C), dim3(kChainThreads), 0, st, a,
+                     (int*)ctx->tmp_count.p);
+  EDHMC_TRY(check_launch(ctx, "lp_load"));
+  int bad = 0;
+  if (hipMemcpyAsync(&bad, ctx->tmp_count.p, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return set_err(ctx, EDHMC_EHIP, "logp: copy failed");
+  if (bad) return set_err(ctx, EDHMC_ENONFINITE, "positions contain %d non-finite rows", bad);
+  EDHMC_TRY(glm_forward_ll(ctx, st));
+  hipLaunchKernelGGL(logp_out_kernel, dim3(C), dim3(kChainThreads), 0, st, a);
+  return check_launch(ctx, "logp_out");
 }
 
 int edhmc_transition(edhmc_ctx* ctx, float eps, int32_t n_steps, const float* r0, const double* u,
@@ -1088,8 +1208,11 @@ int edhmc_sample_moments(edhmc_ctx* ctx, const void* params, int32_t params_dtyp
   return check_launch(ctx, "sample_moments");
 }
 
-int edhmc_record_pairs(edhmc_ctx* ctx, float* W_buf, float* G_buf, int64_t capacity) {
+int edhmc_record_pairs(edhmc_ctx* ctx, float* W_buf, float* G_buf, int64_t capacity,
+                       int32_t stride) {
   if (!ctx) return EDHMC_EINVAL;
+  if (stride < 1) return set_err(ctx, EDHMC_EINVAL, "stride must be >= 1");
+  ctx->rec_stride = stride;
   if (W_buf && (!G_buf || capacity <= 0))
     return set_err(ctx, EDHMC_EINVAL, "recording needs both buffers and a positive capacity");
   ctx->rec_W = W_buf;

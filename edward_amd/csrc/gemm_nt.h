@@ -122,6 +122,7 @@ struct EpiPos {
   int m0, n0;     // global row / column of this wave's first accumulator block
   int lane, wm, wn, tm, tn, ks;
   char* smem;     // free LDS (>= the main loop's footprint) after a barrier
+  int smem_bytes; // its size
 };
 
 template <typename T, typename Cfg, int A_PLANES, typename Epi>
@@ -145,25 +146,29 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
   const int s_begin = ks * g.stages_per_split;
   const int s_end = min(g.k_stages, s_begin + g.stages_per_split);
 
-  const char* src[PER_T];
-  int64_t sstride[PER_T];
-  int dst[PER_T];
-#pragma unroll
-  for (int q = 0; q < PER_T; ++q) {
+  // Chunk q of a stage (16 bytes) is row (tid + q NT) / CPR, column
+  // (tid + q NT) % CPR of the combined [A planes | B] tile.  Whether a chunk
+  // is in A or B depends only on q (AR is a multiple of NT / CPR), so the
+  // address selection below resolves at compile time after unrolling.
+  static_assert((AR * CPR) % NT == 0 || CHUNKS <= NT, "A/B split must fall on a chunk round");
+  const char* a_base = g.A + (int64_t)(tm * BM) * g.lda_b;
+  const char* b_base = g.B + (int64_t)(tn * BN) * g.ldb_b;
+  auto chunk_src = [&](int q, int s) -> const char* {
     int c = tid + q * NT;
     if (c >= CHUNKS) c = CHUNKS - 1;  // duplicate the last chunk (same data)
-    int row = c / CPR, col = c - row * CPR;
+    const int row = c / CPR, col = c - row * CPR;
     if (row < AR) {
-      int p = row / BM, r = row - p * BM;
-      src[q] = g.A + p * g.a_plane_b + (int64_t)(tm * BM + r) * g.lda_b + col * 16;
-      sstride[q] = g.a_stage_b;
-    } else {
-      int r = row - AR;
-      src[q] = g.B + (int64_t)(tn * BN + r) * g.ldb_b + col * 16;
-      sstride[q] = g.b_stage_b;
+      const int p = row / BM, r = row - p * BM;
+      return a_base + p * g.a_plane_b + (int64_t)r * g.lda_b + (int64_t)s * g.a_stage_b + col * 16;
     }
-    dst[q] = row * kRowStride + col * 16;
-  }
+    return b_base + (int64_t)(row - AR) * g.ldb_b + (int64_t)s * g.b_stage_b + col * 16;
+  };
+  auto chunk_dst = [&](int q) -> int {
+    int c = tid + q * NT;
+    if (c >= CHUNKS) c = CHUNKS - 1;
+    const int row = c / CPR, col = c - row * CPR;
+    return row * kRowStride + col * 16;
+  };
 
   f32x16_t acc[WM][WN];
 #pragma unroll
@@ -199,15 +204,23 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
   };
 
   // Two register sets: CUR holds stage s+1 (stored to LDS this iteration),
-  // NXT receives stage s+2.  Written out (no arrays passed by reference) so
-  // the sets stay in VGPRs.
-  uint4 r0[PER_T], r1[PER_T];
+  // NXT receives stage s+2.  Each set is one vector-register tuple
+  // (ext_vector of 4 PER_T dwords), never an array, so it stays in VGPRs.
+  typedef uint32_t PreT __attribute__((ext_vector_type(4 * PER_T)));
+  PreT r0, r1;
 #define EDHMC_GLOAD(PRE, S)                                              \
-  _Pragma("unroll") for (int q = 0; q < PER_T; ++q)                      \
-      PRE[q] = *(const uint4*)(src[q] + (int64_t)(S) * sstride[q]);
+  _Pragma("unroll") for (int q = 0; q < PER_T; ++q) {                    \
+    const uint4 x_ = *(const uint4*)chunk_src(q, (S));                   \
+    PRE[4 * q] = x_.x; PRE[4 * q + 1] = x_.y;                            \
+    PRE[4 * q + 2] = x_.z; PRE[4 * q + 3] = x_.w;                        \
+  }
 #define EDHMC_SSTORE(PRE, BUFI)                                          \
-  _Pragma("unroll") for (int q = 0; q < PER_T; ++q)                      \
-      *(uint4*)(smem + (BUFI) * BUF + dst[q]) = PRE[q];
+  _Pragma("unroll") for (int q = 0; q < PER_T; ++q) {                    \
+    uint4 x_;                                                            \
+    x_.x = PRE[4 * q]; x_.y = PRE[4 * q + 1];                            \
+    x_.z = PRE[4 * q + 2]; x_.w = PRE[4 * q + 3];                        \
+    *(uint4*)(smem + (BUFI) * BUF + chunk_dst(q)) = x_;                  \
+  }
 #define EDHMC_STEP(S, CUR, NXT)                                          \
   {                                                                      \
     const int buf_ = ((S) - s_begin) & 1;                                \
@@ -237,6 +250,7 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
   pos.n0 = tn * BN + wn * 32 * WN;
   pos.lane = lane; pos.wm = wm; pos.wn = wn; pos.tm = tm; pos.tn = tn; pos.ks = ks;
   pos.smem = smem;
+  pos.smem_bytes = 2 * BUF;
   epi.template apply<Cfg>(acc, pos);
 }
 

@@ -18,6 +18,19 @@ def _dev_tensor(x, dtype, device):
   return torch.as_tensor(np.asarray(x), dtype=dtype, device=device).contiguous()
 
 
+def jittered_step(step_size, jitter, seed, counter):
+  """Step size of transition `counter`: step_size * U(1 - jitter, 1 + jitter),
+  a deterministic function of (seed, counter).  A fixed eps L makes posterior
+  directions whose leapfrog rotation over L steps is a multiple of 2 pi
+  return to their start every transition; drawing eps afresh each transition
+  (Neal 2011, "MCMC using Hamiltonian dynamics", sec. 5.4.2) removes that,
+  and keeps the chain exact (the draw is independent of the state)."""
+  if not jitter:
+    return float(step_size)
+  u = np.random.default_rng([int(seed) & 0xFFFFFFFF, int(counter)]).random()
+  return float(step_size) * (1.0 + float(jitter) * (2.0 * u - 1.0))
+
+
 def require_gpu():
   if not torch.cuda.is_available():
     raise RuntimeError('edward_amd needs a ROCm GPU (MI355X/gfx950); none is visible')
@@ -133,6 +146,18 @@ class GLMEngine(Context):
     self.check(self.lib.edhmc_logp_grad(self.h, L.ptr(W), C, L.ptr(g), L.ptr(lp), self.stream))
     return lp, g
 
+  def logp(self, W):
+    """Log joint only (the forward pass of the Metropolis test) at W [C, D]."""
+    W = _dev_tensor(W, torch.float32, self.device)
+    if W.dim() == 1:
+      W = W[None]
+    C = int(W.shape[0])
+    lp = torch.empty(C, dtype=torch.float64, device=self.device)
+    if self.C != C:
+      self.C = C
+    self.check(self.lib.edhmc_logp(self.h, L.ptr(W), C, L.ptr(lp), self.stream))
+    return lp
+
   # -- sampling ------------------------------------------------------------
   def transition(self, step_size, n_steps, momenta=None, uniforms=None, seed=0, counter=0,
                  sample_out=None, n_accept=None, accepted_out=None):
@@ -167,12 +192,14 @@ class GLMEngine(Context):
     self.check(self.lib.edhmc_divergences(self.h, L.ptr(out), self.stream))
     return out
 
-  def record_pairs(self, W_buf=None, G_buf=None):
-    """Record (position, exact gradient) at every inner leapfrog point of the
-    next plain transitions into W_buf/G_buf [capacity, D]; None stops."""
+  def record_pairs(self, W_buf=None, G_buf=None, stride=1):
+    """Record (position, exact gradient) at every `stride`-th inner leapfrog
+    point of the next plain transitions into W_buf/G_buf [capacity, D];
+    None stops."""
     cap = 0 if W_buf is None else int(W_buf.shape[0])
     self._rec = (W_buf, G_buf)   # keep the buffers alive while recording
-    self.check(self.lib.edhmc_record_pairs(self.h, L.ptr(W_buf), L.ptr(G_buf), cap))
+    self.check(self.lib.edhmc_record_pairs(self.h, L.ptr(W_buf), L.ptr(G_buf), cap,
+                                           int(stride)))
 
   def record_count(self):
     return int(self.lib.edhmc_record_count(self.h))

@@ -233,6 +233,8 @@ class HMC(MonteCarlo):
                ``ed.set_seed`` value, else 0)
     surrogate  a trained ``edward_amd.surrogate.Surrogate`` to drive the
                leapfrog (the exact log joint still decides acceptance)
+    step_size_jitter  j in [0, 1): transition t uses step_size * U(1-j, 1+j)
+               (default 0: the reference's fixed step size)
   """
 
   def __init__(self, *args, **kwargs):
@@ -247,6 +249,9 @@ class HMC(MonteCarlo):
     seed = kwargs.pop('seed', None)
     self.seed = util.get_seed() if seed is None else int(seed)
     self.surrogate = kwargs.pop('surrogate', None)
+    self.step_size_jitter = float(kwargs.pop('step_size_jitter', 0.0))
+    if not 0.0 <= self.step_size_jitter < 1.0:
+      raise ValueError('step_size_jitter must be in [0, 1)')
     return super(HMC, self).initialize(*args, **kwargs)
 
   def build_update(self):
@@ -275,11 +280,13 @@ class HMC(MonteCarlo):
     need = max(t - 1, 0)
     if self._state_t != need:
       self.engine.set_state(self.program.gather(need))
+    from edward_amd.engine import jittered_step
+    eps = jittered_step(self.step_size, self.step_size_jitter, self.seed, t)
     if self.surrogate is None:
-      self.engine.transition(self.step_size, self.n_steps, seed=self.seed, counter=t,
+      self.engine.transition(eps, self.n_steps, seed=self.seed, counter=t,
                              sample_out=self._row, n_accept=self._n_accept)
     else:
-      self.surrogate.transition(self.step_size, self.n_steps, seed=self.seed, counter=t,
+      self.surrogate.transition(eps, self.n_steps, seed=self.seed, counter=t,
                                 sample_out=self._row, n_accept=self._n_accept)
     self.program.scatter(t, self._row)
     self._state_t = t

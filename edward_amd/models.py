@@ -166,8 +166,8 @@ def _param_shape(p):
 def _param_dtype(p):
   if isinstance(p, RandomVariable) or isinstance(p, Placeholder):
     return p.dtype
-  if isinstance(p, Node):
-    return None
+  if isinstance(p, Node) or isinstance(p, (bool, int, float)):
+    return None   # Python scalars take the default float32, as TF converts them
   a = _as_array(p)
   return a.dtype.name if a.dtype.kind == 'f' else None
 

@@ -82,7 +82,6 @@ class Surrogate(object):
     o = self._offsets()
     W3 = flat[o['W3']:o['b3']].reshape(self.D, self.H + self.D)
     W3[:, self.H:] = A.T
-    W3[:, :self.H] = 0.0
     flat[o['b3']:o['b3'] + self.D] = 0.0
     self.set_params(flat)
     self.step_count = 0
@@ -146,18 +145,41 @@ class Surrogate(object):
         float(beta2), float(eps), self.step_count, L.ptr(loss), self.engine.stream))
     return loss
 
+  def validation_loss(self, W, G):
+    """Mean loss over (W, G) in batches of max_batch (no parameter update)."""
+    tot, n = 0.0, 0
+    for s in range(0, int(W.shape[0]), self.max_batch):
+      Wc, Gc = W[s:s + self.max_batch], G[s:s + self.max_batch]
+      loss, _ = self.loss_grad(Wc, Gc)
+      tot += loss.item() * int(Wc.shape[0])
+      n += int(Wc.shape[0])
+    return tot / max(n, 1)
+
   def fit(self, W, G, epochs=20, batch=4096, lr=1e-3, lr_final=1e-4, seed=0, normalise=True,
-          log=None):
+          log=None, val_frac=0.0):
     """Adam on the pairs (W [P, D], G [P, D], device tensors), shuffled every
-    epoch, learning rate decayed geometrically from lr to lr_final.
-    Returns the mean loss of each epoch."""
+    epoch, learning rate decayed geometrically from lr to lr_final.  With
+    val_frac > 0 that fraction of the pairs is held out and the weights of
+    the epoch with the lowest held-out loss are kept (early stopping).
+    Returns the mean training loss of each epoch."""
     W = W.to(self.engine.device, torch.float32).contiguous()
     G = G.to(self.engine.device, torch.float32).contiguous()
+    Wv = Gv = None
+    if val_frac > 0:
+      gen0 = torch.Generator(device=self.engine.device)
+      gen0.manual_seed(int(seed) + 1)
+      perm0 = torch.randperm(int(W.shape[0]), generator=gen0, device=self.engine.device)
+      nv = max(1, int(val_frac * int(W.shape[0])))
+      Wv, Gv = W.index_select(0, perm0[:nv]), G.index_select(0, perm0[:nv])
+      W, G = W.index_select(0, perm0[nv:]), G.index_select(0, perm0[nv:])
     P = int(W.shape[0])
     batch = min(int(batch), self.max_batch, P)
     if normalise:
       self.normalise_from(W, G)
       self.init_skip_least_squares(W, G)
+    best = None
+    if Wv is not None:
+      best = (self.validation_loss(Wv, Gv), self.get_params().clone())
     gen = torch.Generator(device=self.engine.device)
     gen.manual_seed(int(seed))
     n_batches = P // batch
@@ -176,8 +198,19 @@ class Surrogate(object):
         step += 1
       ep_loss = torch.cat(acc).mean().item()
       losses.append(ep_loss)
+      msg = 'surrogate epoch %d loss %.5f' % (ep, ep_loss)
+      if best is not None:
+        vl = self.validation_loss(Wv, Gv)
+        msg += ' held-out %.5f' % vl
+        if vl < best[0]:
+          best = (vl, self.get_params().clone())
       if log:
-        log('surrogate epoch %d loss %.5f' % (ep, ep_loss))
+        log(msg)
+    if best is not None:
+      steps = self.step_count
+      self.set_params(best[1].cpu().numpy())
+      self.step_count = steps
+      self.best_validation_loss = best[0]
     return losses
 
   # -- sampling ------------------------------------------------------------

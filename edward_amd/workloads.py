@@ -109,7 +109,10 @@ def hier_groups(seed=0, G=2048, D=64, C=512, mean_size=400):
   eta = a[group] + X.astype(np.float64) @ w
   y = (rng.random(N) < _sigmoid(eta)).astype(np.float32)
   dim = G + 2 + D
-  theta0 = (rng.normal(size=(C, dim)) * 0.1).astype(np.float32)
+  # chains start near the generating values (the posterior of w has sd ~1e-3
+  # at this N, so a cold start spends hundreds of transitions in transit)
+  truth = np.concatenate([a, [mu, log_sigma], w])
+  theta0 = (truth[None, :] + 0.002 * rng.normal(size=(C, dim))).astype(np.float32)
   return Workload(name='hier_groups', G=G, D=D, N=N, C=C, dim=dim, n_steps=16, T=1000,
-                  step_size=0.005, seed=seed, X=X, y=y, offsets=offsets, theta0=theta0,
+                  step_size=0.001, seed=seed, X=X, y=y, offsets=offsets, theta0=theta0,
                   truth=dict(mu=mu, log_sigma=log_sigma, a=a, w=w))

@@ -92,6 +92,13 @@ int edhmc_get_state(edhmc_ctx* ctx, float* W_out, double* logp_out, float* grad_
 int edhmc_logp_grad(edhmc_ctx* ctx, const float* W, int32_t C, float* grad_out,
                     double* logp_out, void* stream);
 
+/* Log joint only at W [C, D] (no gradient): logp_out [C] fp64.  The forward
+ * pass the Metropolis test of hmc.py:104-105 evaluates; in bf16 precision it
+ * uses the hi/lo split positions (two bf16 products) with the FP64
+ * reduction.  Leaves the chain positions untouched, but a C different from
+ * the current chain count discards the chain state. */
+int edhmc_logp(edhmc_ctx* ctx, const float* W, int32_t C, double* logp_out, void* stream);
+
 /* One HMC transition of every chain: momentum draw (hmc.py:88-91),
  * leapfrog with n_steps steps of size eps (hmc.py:94-97,195-210), ratio and
  * accept (hmc.py:100-111), selection, and the sample write + n_accept
@@ -135,16 +142,18 @@ int edhmc_moments_get(edhmc_ctx* ctx, double* mean_out, double* m2_out,
  *   2 GEMM2 (X^T residual, split-K)
  *   3 per-chain leapfrog / accept kernels
  *   4 surrogate MLP GEMMs
- *   5 hierarchical persistent kernel
+ *   5 hierarchical segmented reductions
  * total_ms_out, count_out: host arrays of 6.  Enabling clears the totals. */
 int edhmc_profile_enable(edhmc_ctx* ctx, int32_t enable);
 int edhmc_profile_read(edhmc_ctx* ctx, double* total_ms_out, int64_t* count_out);
 
-/* Record the (position, exact gradient) pair of every inner leapfrog point of
- * the following plain transitions into W_buf / G_buf [capacity, D] (rows
- * appended chain-major per leapfrog step) until full; W_buf = NULL stops.
- * These pairs train the gradient surrogate. */
-int edhmc_record_pairs(edhmc_ctx* ctx, float* W_buf, float* G_buf, int64_t capacity);
+/* Record the (position, exact gradient) pair of every `stride`-th inner
+ * leapfrog point (steps k = stride, 2 stride, ... < n_steps) of the following
+ * plain transitions into W_buf / G_buf [capacity, D] (rows appended
+ * chain-major per leapfrog step) until full; W_buf = NULL stops.  These pairs
+ * train the gradient surrogate. */
+int edhmc_record_pairs(edhmc_ctx* ctx, float* W_buf, float* G_buf, int64_t capacity,
+                       int32_t stride);
 /* Rows recorded by the current (or the last stopped) recording. */
 int64_t edhmc_record_count(edhmc_ctx* ctx);
 
@@ -231,7 +240,9 @@ int edhmc_transition_surrogate(edhmc_ctx* ctx, float eps, int32_t n_steps, const
  * Latent layout per chain: [a (G), mu, log_sigma, w (D)].  In Edward:
  * Bernoulli(logits=tf.gather(a, group) + ed.dot(X, w)); the gather gradient
  * is the IndexedSlices case of tests/inferences/hmc_test.py:105-123 and
- * hmc.py:203,208.  One persistent cooperative launch runs all n_steps.
+ * hmc.py:203,208.  Each gradient pass is the fp32 GLM GEMM pair with the
+ * group intercept gathered into the GEMM1 epilogue, plus a ragged segmented
+ * reduction of the residuals for the intercept gradients.
  * ------------------------------------------------------------------------ */
 int edhmc_bind_hier(edhmc_ctx* ctx, const float* X, const float* y, const int64_t* offsets,
                     int32_t G, int64_t N, int32_t D, void* stream);

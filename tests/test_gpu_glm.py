@@ -86,6 +86,22 @@ def test_logp_grad_bf16(N, D, C):
   _check_logp_grad(lp, g, lp_ref, g_ref, bf16=True)
 
 
+@pytest.mark.parametrize('precision', ['fp32', 'bf16'])
+def test_forward_only_logp(precision):
+  """edhmc_logp (no gradient, the Metropolis test's forward) equals the log
+  joint of edhmc_logp_grad and the oracle; fp32 bar as above, bf16 rtol 1e-5
+  (X rounded to bf16 on both sides)."""
+  from edward_amd import workloads as W
+  wl = W.logreg(3000, 70, 150, 0.01, 10, 10, seed=15, bf16=(precision == 'bf16'))
+  eng = _engine(wl, precision=precision)
+  W0 = (np.random.default_rng(16).normal(size=(150, 70)) * 0.2).astype(np.float32)
+  lp = eng.logp(W0).cpu().numpy()
+  lp_ref, _ = _oracle(wl).logp_grad(W0.astype(np.float64))
+  np.testing.assert_allclose(lp, lp_ref, rtol=1e-5 if precision == 'bf16' else 1e-6)
+  lp2, _ = eng.logp_grad(W0)
+  np.testing.assert_allclose(lp, lp2.cpu().numpy(), rtol=1e-9)
+
+
 def test_nonfinite_data_and_positions_raise():
   from edward_amd._lib import NonFiniteError
   wl = _small(7, 300, 5, 1)

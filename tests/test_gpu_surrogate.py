@@ -82,29 +82,66 @@ def test_adam_step_matches_oracle_on_the_same_gradient():
   np.testing.assert_allclose(p1, want, atol=1e-5, rtol=1e-5)
 
 
-def test_training_learns_a_nonlinear_gradient_field():
-  """Field -P (w - mu) + 2 tanh(3 (w - mu)): the least-squares start of the
-  linear skip captures the first term; the hidden layers must learn the
-  second.  The trained surrogate reproduces the field on fresh points to 3%."""
-  D = 32
-  _, eng = _engine(D=D)
-  sur = _surrogate(eng, H=256, B=1024, seed=5)
-  rng = np.random.default_rng(6)
+def _field_problem(D=32, seed=6, amp=1.0, freq=2.0):
+  rng = np.random.default_rng(seed)
   A = rng.normal(size=(D, D)) / math.sqrt(D)
   P = A @ A.T + np.eye(D)
   mu = rng.normal(size=D)
   cov = np.linalg.inv(P)
 
   def field(w):
-    return -(w - mu) @ P + 2.0 * np.tanh(3.0 * (w - mu))
-  Wtr = rng.multivariate_normal(mu, cov, size=20000)
-  losses = sur.fit(torch.tensor(Wtr, dtype=torch.float32),
-                   torch.tensor(field(Wtr), dtype=torch.float32),
-                   epochs=40, batch=1024, lr=2e-3, lr_final=1e-4)
-  assert losses[-1] < 0.5 * losses[0]
+    return -(w - mu) @ P + amp * np.tanh(freq * (w - mu))
+  return rng, mu, cov, field
+
+
+def test_adam_training_matches_oracle_step_by_step():
+  """30 Adam steps on the same batches from the same start: the GPU losses
+  follow the FP64 oracle's to bf16 accuracy (5%), and the weights end within
+  2% (norm-wise) of the oracle's."""
+  D, H, B = 32, 128, 512
+  _, eng = _engine(D=D)
+  sur = _surrogate(eng, H=H, B=B, seed=5)
+  rng, mu, cov, field = _field_problem(D)
+  Wtr = rng.multivariate_normal(mu, cov, size=B * 6)
+  Gtr = field(Wtr)
+  norm = {'in_shift': Wtr.mean(0), 'in_scale': 1 / Wtr.std(0), 'out_shift': Gtr.mean(0),
+          'out_scale': Gtr.std(0)}
+  _set_norm(sur, norm)
+  flat = sur.get_params().cpu().numpy().astype(np.float64)
+  m = np.zeros_like(flat)
+  v = np.zeros_like(flat)
+  Wt = torch.tensor(Wtr, dtype=torch.float32, device='cuda')
+  Gt = torch.tensor(Gtr, dtype=torch.float32, device='cuda')
+  gl, ol = [], []
+  for step in range(1, 31):
+    sl = slice(((step - 1) % 6) * B, ((step - 1) % 6 + 1) * B)
+    gl.append(sur.train_step(Wt[sl].contiguous(), Gt[sl].contiguous(), lr=3e-3).item())
+    loss, g = MO.loss_and_grad(MO.unflatten(flat, D, H), Wtr[sl], Gtr[sl], norm)
+    ol.append(loss)
+    flat, m, v = MO.adam(flat, MO.flatten(g), m, v, step, 3e-3)
+  np.testing.assert_allclose(gl, ol, rtol=5e-2)
+  assert ol[-1] < 0.5 * ol[0]
+  assert _rel(sur.get_params().cpu().numpy(), flat) < 2e-2
+
+
+def test_hidden_layers_learn_beyond_the_linear_skip():
+  """Field -P (w - mu) + tanh(2 (w - mu)): the least-squares skip captures the
+  linear part; training must cut the remaining error by 40% (the FP64
+  oracle trained the same way reaches 0.47 of the linear-only error)."""
+  D = 32
+  _, eng = _engine(D=D)
+  sur = _surrogate(eng, H=256, B=1024, seed=5)
+  rng, mu, cov, field = _field_problem(D)
+  Wtr = torch.tensor(rng.multivariate_normal(mu, cov, size=20000), dtype=torch.float32)
+  Gtr = torch.tensor(field(Wtr.double().numpy()), dtype=torch.float32)
   Wte = rng.multivariate_normal(mu, cov, size=2000)
-  pred = sur.forward(torch.tensor(Wte, dtype=torch.float32)).cpu().numpy()
-  assert _rel(pred, field(Wte)) < 0.03
+  sur.normalise_from(Wtr.cuda(), Gtr.cuda())
+  sur.init_skip_least_squares(Wtr.cuda(), Gtr.cuda())
+  err_linear = _rel(sur.forward(torch.tensor(Wte, dtype=torch.float32)).cpu().numpy(),
+                    field(Wte))
+  sur.fit(Wtr, Gtr, epochs=80, batch=1024, lr=5e-3, lr_final=1e-4)
+  err = _rel(sur.forward(torch.tensor(Wte, dtype=torch.float32)).cpu().numpy(), field(Wte))
+  assert err < 0.6 * err_linear, (err_linear, err)
 
 
 def test_surrogate_trajectory_matches_oracle_leapfrog():
