@@ -56,16 +56,33 @@ __global__ void group_index_kernel(const int64_t* __restrict__ off, int G, int* 
 }
 
 // grad_a[c, g] = sum_{n in group g} R[c, n]; R^T is fp32 K-blocked
-// ([N_pad/16][C_pad][16]).
+// ([N_pad/16][C_pad][16]: 16 rows of one chain are 64 contiguous bytes).  One
+// workgroup per group; four lanes per chain each load a float4 (rows
+// 4q..4q+3 of a 16-row block), so a wave reads 16 chains x 64 B = 1 KiB
+// contiguous per block.  The four lane sums are combined by lane shuffles in
+// a fixed order (deterministic).
 __global__ void __launch_bounds__(256)
 seg_sum_kernel(const float* __restrict__ Rt, const int64_t* __restrict__ off, int C, int C_pad,
                int G, float* __restrict__ ga) {
   const int g = blockIdx.x;
   const int64_t b = off[g], e = off[g + 1];
-  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+  const int q = threadIdx.x & 3;
+  const int64_t kb0 = b >> 4, kb1 = e > b ? ((e - 1) >> 4) : kb0 - 1;
+  for (int c0 = 0; c0 < C; c0 += 64) {
+    const int c = c0 + (threadIdx.x >> 2);
     float s = 0.0f;
-    for (int64_t n = b; n < e; ++n) s += Rt[((n >> 4) * C_pad + c) * 16 + (n & 15)];
-    ga[(int64_t)c * G + g] = s;
+    if (c < C) {
+#pragma unroll 4
+      for (int64_t kb = kb0; kb <= kb1; ++kb) {
+        const f32x4_t v = *reinterpret_cast<const f32x4_t*>(Rt + (kb * C_pad + c) * 16 + 4 * q);
+        const int64_t n0 = kb * 16 + 4 * q;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s += (n0 + i >= b && n0 + i < e) ? v[i] : 0.0f;
+      }
+    }
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    if (c < C && q == 0) ga[(int64_t)c * G + g] = s;
   }
 }
 
