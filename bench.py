@@ -59,9 +59,12 @@ def parse():
   ap.add_argument('--burn-in', type=int, default=40,
                   help='max plain bf16 burn-in transitions (timed, part of the method)')
   ap.add_argument('--plain-steps', type=int, default=8, help='timed plain fp32 transitions')
-  ap.add_argument('--workload', default='large', choices=['large', 'mid', 'small', 'hier'],
+  ap.add_argument('--workload', default='large',
+                  choices=['large', 'mid', 'small', 'hier', 'copy'],
                   help="'hier': the hierarchical logistic regression (G=2048 groups of "
-                       "1+Poisson(400) rows, D=64, 512 chains, L=16), its own JSON line")
+                       "1+Poisson(400) rows, D=64, 512 chains, L=16), its own JSON line; "
+                       "'copy': 1 GiB device-to-device copies, the byte-counter calibration "
+                       "of HBM traffic profiles")
   ap.add_argument('--hidden', type=int, default=1024)
   ap.add_argument('--epochs', type=int, default=12)
   ap.add_argument('--lr', type=float, default=3e-3)
@@ -172,6 +175,25 @@ def bench_hier(args):
   print(json.dumps(out))
 
 
+def bench_copy(args):
+  """Copies of a 1 GiB buffer: a kernel of known traffic (1 GiB read and
+  1 GiB written per launch) against which the profiler's byte counters are
+  calibrated."""
+  import torch
+  n = 1 << 28
+  a = torch.ones(n, device='cuda')
+  b = torch.empty_like(a)
+  torch.cuda.synchronize()
+  t0 = time.time()
+  for _ in range(args.steps):
+    b.copy_(a)
+  torch.cuda.synchronize()
+  secs = time.time() - t0
+  print(json.dumps({'metric': 'device copy bandwidth', 'value': 2.0 * 4 * n * args.steps / secs
+                    / 1e9, 'unit': 'GB/s', 'bytes_read_per_launch': 4 * n,
+                    'bytes_written_per_launch': 4 * n, 'launches': args.steps}))
+
+
 def main():
   args = parse()
   if args.gpus != 1:
@@ -181,6 +203,8 @@ def main():
     sys.exit(2)
   if args.workload == 'hier':
     return bench_hier(args)
+  if args.workload == 'copy':
+    return bench_copy(args)
   import torch
   from edward_amd import diagnostics
   from edward_amd.engine import GLMEngine
@@ -320,8 +344,12 @@ def main():
   kinds = {k: v for k, v in prof.items() if v[1] > 0}
   dom = max(kinds, key=lambda k: kinds[k][0])
   ms, cnt = kinds[dom]
+  issued = None
   if dom == 'gemm_ll':
-    flop = 4.0 * N * D * C          # two bf16 products (hi/lo split positions)
+    # algorithmic work of the exact forward: eta = W X^T, 2 N D C flop; the
+    # kernel issues twice that on the matrix cores (hi and lo bf16 planes of W)
+    flop = 2.0 * N * D * C
+    issued = 2.0 * flop
   elif dom == 'mlp':
     H = args.hidden
     flop = 2.0 * C * (D * H + H * H + H * D) / 3.0   # mean over the 3 layer GEMMs
@@ -330,7 +358,8 @@ def main():
   achieved = flop / (ms / cnt * 1e-3)
   roof = {'bound': 'mfma', 'kernel': dom, 'achieved': achieved / 1e12,
           'peak': PEAK['bf16'] / 1e12, 'unit': 'TFLOP/s', 'frac': achieved / PEAK['bf16'],
-          'traffic': None, 'avg_launch_ms': ms / cnt,
+          'traffic': None, 'avg_launch_ms': ms / cnt, 'flop_per_launch': flop,
+          'mfma_issued_frac': None if issued is None else issued / (ms / cnt * 1e-3) / PEAK['bf16'],
           'per_transition_ms': {k: v[0] / K for k, v in kinds.items()}}
 
   cpu = None

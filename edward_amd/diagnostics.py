@@ -1,15 +1,15 @@
 """Effective sample size from device sample stores (no reference counterpart:
 Edward reports only the acceptance rate, ``monte_carlo.py:152-158``).
 
-The lag sums run on the GPU (``edhmc_autocov``); the truncation is done here:
-the autocorrelations pooled over chains, rho_t = mean_m acov_m(t) /
-mean_m acov_m(0), summed with Geyer's (1992) initial positive, monotone
-sequence, tau floored at 1/log10(M n), ESS = M n / tau.  The within-chain
-normalisation (``between=False``, the default) credits only what the
-transitions themselves decorrelate: chains that never move score 0 even when
-their starting points are spread over the posterior.  ``between=True`` gives
-the multi-chain estimator of Gelman et al. (BDA3, sec. 11.5), whose
-denominator also counts the spread of the chain means."""
+The lag sums run on the GPU (``edhmc_autocov``); the truncation is done here.
+``between=True`` (the default) is the multi-chain estimator of Gelman et al.
+(BDA3, sec. 11.5): rho_t = 1 - (W - mean_m acov_m(t)) / var+, with var+ the
+within-chain variance plus the spread of the chain means, summed with Geyer's
+(1992) initial positive, monotone sequence, tau floored at 1/log10(M n),
+ESS = M n / tau.  ``between=False`` leaves the spread of the chain means out,
+so it credits only what the transitions themselves decorrelate: chains that
+never move score 0 even when their starting points are spread over the
+posterior."""
 
 import math
 

@@ -282,3 +282,24 @@ def test_large_workload_bf16_logp_grad_and_directional_derivative():
   fd = (lpp - lpm).cpu().numpy() / (2 * h)
   gv = (g.cpu().numpy().astype(np.float64) * v).sum(1)
   assert np.median(np.abs(fd - gv) / np.abs(gv)) < 1e-2
+
+
+@pytest.mark.parametrize('between', [True, False])
+def test_device_ess_matches_oracle(between):
+  """ESS from the device autocovariances against the oracle's FFT-based
+  multi-chain estimator on AR(1) chains of several correlations."""
+  from edward_amd import diagnostics
+  wl = _small(3, 200, 4, 2)
+  eng = _engine(wl)
+  rng = np.random.default_rng(40)
+  T, C, D = 400, 8, 6
+  rho = np.array([0.0, 0.3, 0.6, 0.9, 0.95, 0.5])
+  x = np.zeros((T, C, D))
+  x[0] = rng.normal(size=(C, D)) / np.sqrt(1 - rho ** 2)
+  for t in range(1, T):
+    x[t] = rho * x[t - 1] + rng.normal(size=(C, D))
+  x = x + rng.normal(size=(1, C, D)) * 0.2    # chain offsets (between-chain spread)
+  xf = x.astype(np.float32)
+  dev = diagnostics.ess(eng, torch.as_tensor(xf, device='cuda'), between=between)
+  ref = np.array([O.ess(xf[:, :, d].astype(np.float64).T, between=between) for d in range(D)])
+  np.testing.assert_allclose(dev, ref, rtol=1e-6)
