@@ -93,6 +93,8 @@ struct edhmc_ctx {
   int rec_stride = 1;
   // ---- misc scratch ----
   edhmc::DevBuf tmp_count, stats_part;
+  edhmc::DevBuf pred_col;   // posterior-predictive column partials [m_tiles][N_pad]
+  edhmc::DevBuf ll_split;   // log-likelihood range sums [16][C_pad] (fixed-order reduction)
 
   edhmc::Profiler prof;
   edhmc::Surrogate* sur = nullptr;

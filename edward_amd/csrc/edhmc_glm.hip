@@ -65,6 +65,13 @@ int prof_collect(edhmc_ctx* ctx) {
 #ifndef EDHMC_CFG_P2
 #define EDHMC_CFG_P2 1, 4, 2, 2
 #endif
+#ifndef EDHMC_GEMM1_SWZ
+#define EDHMC_GEMM1_SWZ 3
+#endif
+// GEMM1 tile order: 3 = each XCD keeps a slice of the chains (the W planes
+// stay in its L2) and streams X tiles; 1 = each XCD keeps X row tiles and
+// streams all chains.
+constexpr int kGemm1Swz = EDHMC_GEMM1_SWZ;
 using Cfg1 = TileCfg<EDHMC_CFG_G1>;    // GEMM1 (gradient pass, one plane)
 using CfgG2 = TileCfg<EDHMC_CFG_G2>;   // GEMM2 when D_pad % CfgG2::BN == 0
 using Cfg2S = TileCfg<1, 1, 4, 1>;     // 128 x 32, GEMM2 for narrow D
@@ -182,6 +189,128 @@ struct EpiGLM {
 };
 
 // ---------------------------------------------------------------------------
+// Posterior-predictive epilogue (criticism): accumulator element (m, n) =
+// eta of posterior draw m at data row n.  Per element: the log-likelihood of
+// y_n (summed per draw in FP64, as EpiGLM's partials), the mean response
+// (sigmoid(eta) or eta) summed over the tile's valid draws into a per-tile
+// column partial, and optionally the mean response itself into rep[m][n]
+// (replicate_kernel then draws y_rep from it; keeping the RNG out of the
+// epilogue keeps the accumulator loops fully unrolled, in registers).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct EpiPredict {
+  const float* y;
+  int64_t N;
+  LikParams lp;
+  double* llpart;       // [n_tiles][ll_ld]
+  int64_t ll_ld;
+  float* colsum;        // [m_tiles][N_pad] or null
+  int64_t col_ld;       // N_pad
+  float* rep;           // [m_valid][N] or null
+  int m_valid;          // number of draws (rows >= m_valid are padding)
+
+  template <typename Cfg>
+  __device__ void apply(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p) const {
+    constexpr int WM = Cfg::kWM, WN = Cfg::kWN, BM = Cfg::BM, NT = Cfg::kThreads;
+    double* sm = reinterpret_cast<double*>(p.smem);                 // [kWavesN][BM]
+    float* cs = reinterpret_cast<float*>(p.smem + Cfg::kWavesN * BM * 8);  // [kWavesM][BN]
+    const int ml0 = p.m0 - p.tm * BM, nl0 = p.n0 - p.tn * Cfg::BN;
+    float yv[WN], col[WN];
+    bool valid[WN];
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      const int64_t n = p.n0 + 32 * j + (p.lane & 31);
+      valid[j] = n < N;
+      yv[j] = valid[j] ? y[n] : 0.0f;
+      col[j] = 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int mr = 32 * i + acc_row(r, p.lane);
+        const int m = p.m0 + mr;
+        const bool mv = m < m_valid;
+        double llsum = 0.0;
+#pragma unroll
+        for (int j = 0; j < WN; ++j) {
+          const float eta = acc[i][j][r];
+          const int64_t n = p.n0 + 32 * j + (p.lane & 31);
+          const double ll = glm_ll(lp, eta, yv[j]);
+          llsum += valid[j] ? ll : 0.0;
+          float mean;
+          if (lp.lik == LIK_BERNOULLI) {
+            const float e = __expf(-fabsf(eta));
+            const float inv = fast_rcp(1.0f + e);
+            mean = eta >= 0.0f ? inv : e * inv;
+          } else {
+            mean = eta;
+          }
+          col[j] += mv ? mean : 0.0f;
+          // the mean response; replicate_kernel turns it into a draw
+          if (rep && mv && valid[j]) rep[(int64_t)m * N + n] = mean;
+        }
+#pragma unroll
+        for (int off = 16; off > 0; off >>= 1) llsum += __shfl_xor(llsum, off, 64);
+        if ((p.lane & 31) == 0) sm[p.wn * BM + ml0 + mr] = llsum;
+      }
+    // column sums: the two lane halves hold different rows of the same columns
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      col[j] += __shfl_xor(col[j], 32, 64);
+      if (p.lane < 32) cs[p.wm * Cfg::BN + nl0 + 32 * j + p.lane] = col[j];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < BM; t += NT) {
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < Cfg::kWavesN; ++w) s += sm[w * BM + t];
+      llpart[(int64_t)p.tn * ll_ld + p.tm * BM + t] = s;
+    }
+    if (colsum) {
+      for (int t = threadIdx.x; t < Cfg::BN; t += NT) {
+        float s = 0.0f;
+#pragma unroll
+        for (int w = 0; w < Cfg::kWavesM; ++w) s += cs[w * Cfg::BN + t];
+        colsum[(int64_t)p.tm * col_ld + (int64_t)p.tn * Cfg::BN + t] = s;
+      }
+    }
+  }
+};
+
+// y_rep[s][n] from the mean response mu in place: Bernoulli u < mu with a
+// 24-bit Philox uniform u; Normal mu + scale * z (Box-Muller).  Counter
+// (s, n) on stream 1 of `seed`.
+__global__ void __launch_bounds__(256)
+replicate_kernel(float* __restrict__ rep, int S, int64_t N, int lik, float scale, uint64_t seed) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (int64_t)S * N) return;
+  const int s = (int)(i / N);
+  const int64_t n = i - (int64_t)s * N;
+  const u32x4 x = philox_draw(seed, (uint64_t)s, (uint32_t)n, 0u, 1);
+  const float mu = rep[i];
+  if (lik == LIK_BERNOULLI) {
+    const float u = (float)(x.x >> 8) * 5.9604644775390625e-8f;   // [0, 1)
+    rep[i] = u < mu ? 1.0f : 0.0f;
+  } else {
+    float z0, z1;
+    box_muller(x.x, x.y, z0, z1);
+    rep[i] = mu + scale * z0;
+  }
+}
+
+// mean_out[n] = (sum over M tiles of colsum[tm][n]) / S
+__global__ void __launch_bounds__(256)
+colsum_reduce_kernel(const float* __restrict__ part, int m_tiles, int64_t ld, int64_t N, float inv_s,
+                     float* __restrict__ out) {
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.0f;
+  for (int t = 0; t < m_tiles; ++t) s += part[(int64_t)t * ld + n];
+  out[n] = s * inv_s;
+}
+
+// ---------------------------------------------------------------------------
 // Data layout: X [N, D] (fp32 or bf16 in) -> Xr [N_pad, D_pad], Xc [D_pad, N_pad]
 // in the compute type; counts non-finite inputs.
 // ---------------------------------------------------------------------------
@@ -218,17 +347,37 @@ layout_x_kernel(const Tin* __restrict__ X, int64_t N, int D, T* __restrict__ Xr,
 }
 
 // Deterministic reduction of the log-likelihood partials llpart[t][c] over
-// the row tiles t: 64 chains per block, 4 fixed strides of t, fixed order.
+// the row tiles t, in two fixed-order passes: kLLSplit contiguous ranges of t
+// per 64-chain block (4 interleaved strides inside a range, combined in a
+// fixed order), then the kLLSplit range sums in order.  The split keeps
+// thousands of threads in flight when C is small (a 512-chain reduction over
+// 6400 tiles would otherwise run on 8 workgroups).
+constexpr int kLLSplit = 16;
+
 __global__ void __launch_bounds__(256)
-ll_reduce_kernel(const double* __restrict__ llpart, int n_tiles, int C_pad, double* __restrict__ ll) {
+ll_partial_kernel(const double* __restrict__ llpart, int n_tiles, int C_pad,
+                  double* __restrict__ split) {
   __shared__ double part[4][64];
   const int tc = threadIdx.x & 63, tg = threadIdx.x >> 6;
   const int c = blockIdx.x * 64 + tc;
+  const int per = (n_tiles + kLLSplit - 1) / kLLSplit;
+  const int t0 = blockIdx.y * per, t1 = min(n_tiles, t0 + per);
   double s = 0.0;
-  for (int t = tg; t < n_tiles; t += 4) s += llpart[(int64_t)t * C_pad + c];
+  for (int t = t0 + tg; t < t1; t += 4) s += llpart[(int64_t)t * C_pad + c];
   part[tg][tc] = s;
   __syncthreads();
-  if (tg == 0) ll[c] = ((part[0][tc] + part[1][tc]) + part[2][tc]) + part[3][tc];
+  if (tg == 0)
+    split[(int64_t)blockIdx.y * C_pad + c] = ((part[0][tc] + part[1][tc]) + part[2][tc]) + part[3][tc];
+}
+
+__global__ void __launch_bounds__(256)
+ll_final_kernel(const double* __restrict__ split, int C_pad, double* __restrict__ ll) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= C_pad) return;
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < kLLSplit; ++k) s += split[(int64_t)k * C_pad + c];
+  ll[c] = s;
 }
 
 __global__ void pad_copy_kernel(const float* __restrict__ src, int64_t n, float* __restrict__ dst,
@@ -645,6 +794,16 @@ static ChainArgs chain_args(edhmc_ctx* ctx) {
   return a;
 }
 
+// ll[c] = sum over the n_tiles row tiles of llpart[t][c] (fixed order).
+static int ll_reduce(edhmc_ctx* ctx, int n_tiles, hipStream_t st) {
+  hipLaunchKernelGGL(ll_partial_kernel, dim3(ctx->C_pad / 64, kLLSplit), dim3(256), 0, st,
+                     (const double*)ctx->llpart.p, n_tiles, ctx->C_pad, (double*)ctx->ll_split.p);
+  EDHMC_TRY(check_launch(ctx, "ll_partial"));
+  hipLaunchKernelGGL(ll_final_kernel, dim3((ctx->C_pad + 255) / 256), dim3(256), 0, st,
+                     (const double*)ctx->ll_split.p, ctx->C_pad, (double*)ctx->ll.p);
+  return check_launch(ctx, "ll_final");
+}
+
 template <typename T, typename Cfg, int PLANES, bool WR, bool LL>
 static int launch_gemm1(edhmc_ctx* ctx, hipStream_t st, const float* goff = nullptr,
                         int64_t goff_ld = 0, const int* gidx = nullptr) {
@@ -662,7 +821,7 @@ static int launch_gemm1(edhmc_ctx* ctx, hipStream_t st, const float* goff = null
   g.k_stages = ctx->D_pad * eb / kStageBytes;
   g.ksplit = 1;
   g.stages_per_split = g.k_stages;
-  g.swz = 1;
+  g.swz = (kGemm1Swz == 3 && g.m_tiles % 8 == 0) ? 3 : 1;
   ProfScope ps(ctx, WR ? PK_GEMM_GRAD : PK_GEMM_LL, st);
   EpiGLM<T, WR, LL> e;
   e.y = (const float*)ctx->y.p;
@@ -675,13 +834,12 @@ static int launch_gemm1(edhmc_ctx* ctx, hipStream_t st, const float* goff = null
   e.goff = goff;
   e.goff_ld = goff_ld;
   e.gidx = gidx;
+  if (!gemm_offsets_fit<Cfg, PLANES>(g)) return set_err(ctx, EDHMC_EUNSUPPORTED, "GEMM1 tile too large");
   hipLaunchKernelGGL((gemm_nt_kernel<T, Cfg, PLANES, EpiGLM<T, WR, LL>>), dim3(grid_size(g)),
                      dim3(Cfg::kThreads), 0, st, g, e);
   EDHMC_TRY(check_launch(ctx, "gemm1"));
   if (LL) {
-    hipLaunchKernelGGL(ll_reduce_kernel, dim3(ctx->C_pad / 64), dim3(256), 0, st,
-                       (const double*)ctx->llpart.p, g.n_tiles, ctx->C_pad, (double*)ctx->ll.p);
-    EDHMC_TRY(check_launch(ctx, "ll_reduce"));
+    EDHMC_TRY(ll_reduce(ctx, g.n_tiles, st));
   }
   return EDHMC_OK;
 }
@@ -708,6 +866,7 @@ static int launch_gemm2_cfg(edhmc_ctx* ctx, hipStream_t st) {
   e.out = (float*)ctx->Gpart.p;
   e.ldc = ctx->D_pad;
   e.split_b = (int64_t)ctx->C_pad * ctx->D_pad;
+  if (!gemm_offsets_fit<Cfg, 1>(g)) return set_err(ctx, EDHMC_EUNSUPPORTED, "GEMM2 tile too large");
   hipLaunchKernelGGL((gemm_nt_kernel<T, Cfg, 1, EpiStoreF32>), dim3(grid_size(g)),
                      dim3(Cfg::kThreads), 0, st, g, e);
   return check_launch(ctx, "gemm2");
@@ -763,6 +922,55 @@ int glm_write_lp(edhmc_ctx* ctx, hipStream_t st) {
   return check_launch(ctx, "chain_lp");
 }
 
+static int ensure_chains(edhmc_ctx* ctx, int C);
+
+// Posterior-predictive pass over the bound rows for the S draws in z_lp.
+template <typename T, typename Cfg, int PLANES>
+static int launch_predict(edhmc_ctx* ctx, int S, bool want_mean, float* rep, uint64_t seed,
+                          hipStream_t st) {
+  const int eb = sizeof(T);
+  GemmShape g = {};
+  g.A = (const char*)ctx->z_lp.p;
+  g.lda_b = (int64_t)ctx->D_pad * eb;
+  g.a_stage_b = kStageBytes;
+  g.a_plane_b = (int64_t)ctx->C_pad * ctx->D_pad * eb;
+  g.B = (const char*)ctx->Xr.p;
+  g.ldb_b = (int64_t)ctx->D_pad * eb;
+  g.b_stage_b = kStageBytes;
+  g.m_tiles = ctx->C_pad / Cfg::BM;
+  g.n_tiles = (int)(ctx->N_pad / Cfg::BN);
+  g.k_stages = ctx->D_pad * eb / kStageBytes;
+  g.ksplit = 1;
+  g.stages_per_split = g.k_stages;
+  g.swz = 1;
+  if (!gemm_offsets_fit<Cfg, PLANES>(g)) return set_err(ctx, EDHMC_EUNSUPPORTED, "tile too large");
+  if (want_mean) EDHMC_TRY(dev_alloc(ctx, ctx->pred_col, (size_t)g.m_tiles * ctx->N_pad * 4, false));
+  EpiPredict<T> e;
+  e.y = (const float*)ctx->y.p;
+  e.N = ctx->N;
+  e.lp = lik_params(ctx);
+  e.llpart = (double*)ctx->llpart.p;
+  e.ll_ld = ctx->C_pad;
+  e.colsum = want_mean ? (float*)ctx->pred_col.p : nullptr;
+  e.col_ld = ctx->N_pad;
+  e.rep = rep;
+  e.m_valid = S;
+  {
+    ProfScope ps(ctx, PK_GEMM_LL, st);
+    hipLaunchKernelGGL((gemm_nt_kernel<T, Cfg, PLANES, EpiPredict<T>>), dim3(grid_size(g)),
+                       dim3(Cfg::kThreads), 0, st, g, e);
+  }
+  EDHMC_TRY(check_launch(ctx, "predict"));
+  EDHMC_TRY(ll_reduce(ctx, g.n_tiles, st));
+  if (rep) {
+    const int64_t total = (int64_t)S * ctx->N;
+    hipLaunchKernelGGL(replicate_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                       rep, S, ctx->N, ctx->lik, ctx->lik_scale, seed);
+    EDHMC_TRY(check_launch(ctx, "replicate"));
+  }
+  return EDHMC_OK;
+}
+
 static int ensure_chains(edhmc_ctx* ctx, int C) {
   if (C <= 0) return set_err(ctx, EDHMC_EINVAL, "number of chains must be positive, got %d", C);
   if (ctx->C == C && ctx->z.p) return EDHMC_OK;
@@ -798,6 +1006,7 @@ static int ensure_chains(edhmc_ctx* ctx, int C) {
   ctx->n_tiles1 = (int)(ctx->N_pad / std::min(Cfg1::BN, CfgP2::BN));   // llpart rows
   EDHMC_TRY(dev_alloc(ctx, ctx->llpart, (size_t)C_pad * ctx->n_tiles1 * 8, false));
   EDHMC_TRY(dev_alloc(ctx, ctx->ll, (size_t)C_pad * 8));
+  EDHMC_TRY(dev_alloc(ctx, ctx->ll_split, (size_t)kLLSplit * C_pad * 8, false));
   return EDHMC_OK;
 }
 
@@ -903,7 +1112,7 @@ edhmc_ctx::~edhmc_ctx() {
   if (hier) hier_destroy(hier);
   DevBuf* bufs[] = {&Xr, &Xc, &y, &ploc, &pscale, &z, &r, &g, &z0, &g0, &z_lp, &logp, &K0,
                     &flag, &ndiv, &Rt, &Gpart, &llpart, &ll, &mom_mean, &mom_m2, &tmp_count,
-                    &stats_part};
+                    &stats_part, &pred_col, &ll_split};
   for (DevBuf* b : bufs) b->release();
 }
 
@@ -1068,6 +1277,42 @@ int edhmc_logp(edhmc_ctx* ctx, const float* W, int32_t C, double* logp_out, void
   EDHMC_TRY(glm_forward_ll(ctx, st));
   hipLaunchKernelGGL(logp_out_kernel, dim3(C), dim3(kChainThreads), 0, st, a);
   return check_launch(ctx, "logp_out");
+}
+
+int edhmc_predict(edhmc_ctx* ctx, const float* W, int32_t S, float* mean_out, double* loglik_out,
+                  float* rep_out, uint64_t seed, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  if (!ctx->bound) return set_err(ctx, EDHMC_ESTATE, "no model bound");
+  if (!W) return set_err(ctx, EDHMC_EINVAL, "W is NULL");
+  EDHMC_TRY(ensure_chains(ctx, S));
+  EDHMC_TRY(dev_alloc(ctx, ctx->tmp_count, 4));
+  ChainArgs a = chain_args(ctx);
+  a.W_in = W;
+  hipLaunchKernelGGL(lp_load_kernel, dim3(S), dim3(kChainThreads), 0, st, a,
+                     (int*)ctx->tmp_count.p);
+  EDHMC_TRY(check_launch(ctx, "lp_load"));
+  int bad = 0;
+  if (hipMemcpyAsync(&bad, ctx->tmp_count.p, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return set_err(ctx, EDHMC_EHIP, "predict: copy failed");
+  if (bad) return set_err(ctx, EDHMC_ENONFINITE, "draws contain %d non-finite rows", bad);
+  if (ctx->cdt == EDHMC_BF16)
+    EDHMC_TRY((launch_predict<bf16_t, CfgP2, 2>(ctx, S, mean_out != nullptr, rep_out, seed, st)));
+  else
+    EDHMC_TRY((launch_predict<float, Cfg1, 1>(ctx, S, mean_out != nullptr, rep_out, seed, st)));
+  if (mean_out) {
+    const int m_tiles = ctx->C_pad / (ctx->cdt == EDHMC_BF16 ? CfgP2::BM : Cfg1::BM);
+    hipLaunchKernelGGL(colsum_reduce_kernel, dim3((unsigned)((ctx->N + 255) / 256)), dim3(256), 0,
+                       st, (const float*)ctx->pred_col.p, m_tiles, ctx->N_pad, ctx->N,
+                       1.0f / (float)S, mean_out);
+    EDHMC_TRY(check_launch(ctx, "colsum_reduce"));
+  }
+  if (loglik_out && hipMemcpyAsync(loglik_out, ctx->ll.p, (size_t)S * 8, hipMemcpyDeviceToDevice,
+                                   st) != hipSuccess)
+    return set_err(ctx, EDHMC_EHIP, "predict: copy failed");
+  return EDHMC_OK;
 }
 
 int edhmc_transition(edhmc_ctx* ctx, float eps, int32_t n_steps, const float* r0, const double* u,

@@ -32,7 +32,11 @@
 namespace edhmc {
 
 constexpr int kStageBytes = 64;
-constexpr int kRowStride = 80;
+#ifndef EDHMC_LDS_ROW_STRIDE
+#define EDHMC_LDS_ROW_STRIDE 80
+#endif
+constexpr int kRowStride = EDHMC_LDS_ROW_STRIDE;   // LDS bytes per staged row (64 + pad)
+static_assert(kRowStride >= kStageBytes && kRowStride % 16 == 0, "LDS row stride");
 
 struct GemmShape {
   const char* A;      // [planes][M][K]
@@ -46,7 +50,8 @@ struct GemmShape {
   int k_stages;          // K bytes / 64
   int ksplit;            // number of K ranges
   int stages_per_split;
-  int swz;               // 0: plain, 1: group by n tile, 2: group by K split
+  int swz;               // 0: plain, 1: group by n tile, 2: group by K split,
+                         // 3: XCD owns a slice of M tiles (m_tiles % 8 == 0)
 };
 
 __device__ inline bool map_tile(const GemmShape& g, int id, int& tm, int& tn, int& ks) {
@@ -55,6 +60,16 @@ __device__ inline bool map_tile(const GemmShape& g, int id, int& tm, int& tn, in
     int grp = local / g.m_tiles;
     tm = local - grp * g.m_tiles;
     tn = xcd + 8 * grp;
+    ks = 0;
+    return tn < g.n_tiles;
+  }
+  if (g.swz == 3) {
+    // each XCD owns m_tiles / 8 consecutive M tiles (its slice of A stays in
+    // its L2) and walks the N tiles; m_tiles % 8 == 0
+    const int xcd = id & 7, local = id >> 3, mper = g.m_tiles >> 3;
+    const int grp = local / mper;
+    tm = xcd * mper + (local - grp * mper);
+    tn = grp;
     ks = 0;
     return tn < g.n_tiles;
   }
@@ -80,6 +95,26 @@ inline int grid_size(const GemmShape& g) {
   if (g.swz == 2) return g.m_tiles * g.n_tiles * ((g.ksplit + 7) / 8) * 8;
   return g.m_tiles * g.n_tiles * g.ksplit;
 }
+
+// The K loop addresses a chunk as stage base + 32-bit lane offset: the
+// largest offset inside one stage tile must fit (A planes, rows of A and B).
+template <typename Cfg, int A_PLANES>
+inline bool gemm_offsets_fit(const GemmShape& g) {
+  const int64_t a = (int64_t)(A_PLANES - 1) * g.a_plane_b + (int64_t)Cfg::BM * g.lda_b;
+  const int64_t b = (int64_t)Cfg::BN * g.ldb_b;
+  return a < (int64_t)UINT32_MAX && b < (int64_t)UINT32_MAX;
+}
+
+#ifndef EDHMC_PREFETCH_DEPTH
+#define EDHMC_PREFETCH_DEPTH 3
+#endif
+constexpr int kPrefetchDepth = EDHMC_PREFETCH_DEPTH;   // register stages in flight (2 or 3)
+static_assert(kPrefetchDepth == 2 || kPrefetchDepth == 3, "prefetch depth");
+
+// Word 3 of a raw (unswizzled, stride 0) buffer resource on gfx9-family
+// CDNA parts: 32-bit data format, no index, offsets checked against the
+// byte range in word 2.
+constexpr int kBufferRsrcWord3 = 0x00020000;
 
 template <typename T> struct Mfma;
 
@@ -151,17 +186,65 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
   // is in A or B depends only on q (AR is a multiple of NT / CPR), so the
   // address selection below resolves at compile time after unrolling.
   static_assert((AR * CPR) % NT == 0 || CHUNKS <= NT, "A/B split must fall on a chunk round");
+  // When every round of NT chunks lies wholly in A or in B, a chunk's
+  // address is a wave-uniform stage base (scalar registers) plus a per-lane
+  // 32-bit offset fixed for the whole K loop: no per-stage vector address
+  // arithmetic and no 64-bit address registers (which the allocator would
+  // otherwise share with the prefetch registers, serialising loads behind
+  // the LDS stores).  gemm_offsets_fit() checks the offsets fit in 32 bits.
+  constexpr bool kRoundSplit = (AR * CPR) % NT == 0;
   const char* a_base = g.A + (int64_t)(tm * BM) * g.lda_b;
   const char* b_base = g.B + (int64_t)(tn * BN) * g.ldb_b;
-  auto chunk_src = [&](int q, int s) -> const char* {
+  uint32_t loff[PER_T];
+#pragma unroll
+  for (int q = 0; q < PER_T; ++q) {
     int c = tid + q * NT;
     if (c >= CHUNKS) c = CHUNKS - 1;  // duplicate the last chunk (same data)
     const int row = c / CPR, col = c - row * CPR;
     if (row < AR) {
       const int p = row / BM, r = row - p * BM;
-      return a_base + p * g.a_plane_b + (int64_t)r * g.lda_b + (int64_t)s * g.a_stage_b + col * 16;
+      loff[q] = (uint32_t)(p * g.a_plane_b + (int64_t)r * g.lda_b + col * 16);
+    } else {
+      loff[q] = (uint32_t)((int64_t)(row - AR) * g.ldb_b + col * 16);
     }
-    return b_base + (int64_t)(row - AR) * g.ldb_b + (int64_t)s * g.b_stage_b + col * 16;
+  }
+  auto chunk_src = [&](int q, int s) -> const char* {
+    if (kRoundSplit) {
+      const bool in_a = q * NT < AR * CPR;   // compile-time after unrolling
+      const char* base = in_a ? a_base + (int64_t)s * g.a_stage_b : b_base + (int64_t)s * g.b_stage_b;
+      return base + loff[q];
+    }
+    int c = tid + q * NT;
+    if (c >= CHUNKS) c = CHUNKS - 1;
+    const int row = c / CPR;
+    return (row < AR ? a_base + (int64_t)s * g.a_stage_b : b_base + (int64_t)s * g.b_stage_b) +
+           loff[q];
+  };
+  // Per-stage buffer resources (scalar registers): base = the stage's tile
+  // origin in A and in B, range = the tile's extent, so a chunk load is one
+  // buffer_load with a lane offset fixed for the K loop and no vector
+  // address arithmetic.
+  typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+  struct stage_rsrc { __amdgpu_buffer_rsrc_t a, b; };
+  const int a_extent = (int)((A_PLANES - 1) * g.a_plane_b + (int64_t)BM * g.lda_b);
+  const int b_extent = (int)((int64_t)BN * g.ldb_b);
+  auto make_stage_rsrc = [&](int s) -> stage_rsrc {
+    stage_rsrc r;
+    r.a = __builtin_amdgcn_make_buffer_rsrc((void*)(a_base + (int64_t)s * g.a_stage_b), (short)0,
+                                            a_extent, kBufferRsrcWord3);
+    r.b = __builtin_amdgcn_make_buffer_rsrc((void*)(b_base + (int64_t)s * g.b_stage_b), (short)0,
+                                            b_extent, kBufferRsrcWord3);
+    return r;
+  };
+  auto load_chunk = [&](const stage_rsrc& r, int q, int s) -> u32x4_t {
+    if (kRoundSplit) {
+      const bool in_a = q * NT < AR * CPR;   // compile-time after unrolling
+      return __builtin_amdgcn_raw_buffer_load_b128(in_a ? r.a : r.b, (int)loff[q], 0, 0);
+    }
+    const uint4 x = *(const uint4*)chunk_src(q, s);
+    u32x4_t v;
+    v.x = x.x; v.y = x.y; v.z = x.z; v.w = x.w;
+    return v;
   };
   auto chunk_dst = [&](int q) -> int {
     int c = tid + q * NT;
@@ -203,16 +286,21 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
     }
   };
 
-  // Two register sets: CUR holds stage s+1 (stored to LDS this iteration),
-  // NXT receives stage s+2.  Each set is one vector-register tuple
-  // (ext_vector of 4 PER_T dwords), never an array, so it stays in VGPRs.
+  // Register prefetch sets: each is one vector-register tuple (ext_vector of
+  // 4 PER_T dwords), never an array, so it stays in VGPRs.  With depth D the
+  // loads of stage s + D are issued while stage s is computed and stage s + 1
+  // is stored to LDS, so D - 1 stages of compute hide each load's latency.
   typedef uint32_t PreT __attribute__((ext_vector_type(4 * PER_T)));
-  PreT r0, r1;
+  PreT r0, r1, r2;
 #define EDHMC_GLOAD(PRE, S)                                              \
-  _Pragma("unroll") for (int q = 0; q < PER_T; ++q) {                    \
-    const uint4 x_ = *(const uint4*)chunk_src(q, (S));                   \
-    PRE[4 * q] = x_.x; PRE[4 * q + 1] = x_.y;                            \
-    PRE[4 * q + 2] = x_.z; PRE[4 * q + 3] = x_.w;                        \
+  {                                                                      \
+    const stage_rsrc rs_ = make_stage_rsrc(S);                           \
+    _Pragma("unroll") for (int q = 0; q < PER_T; ++q) {                  \
+      const u32x4_t x_ = load_chunk(rs_, q, (S));                        \
+      PRE[4 * q] = x_.x; PRE[4 * q + 1] = x_.y;                          \
+      PRE[4 * q + 2] = x_.z; PRE[4 * q + 3] = x_.w;                      \
+    }                                                                    \
+    __builtin_amdgcn_sched_barrier(0);                                   \
   }
 #define EDHMC_SSTORE(PRE, BUFI)                                          \
   _Pragma("unroll") for (int q = 0; q < PER_T; ++q) {                    \
@@ -221,26 +309,60 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
     x_.z = PRE[4 * q + 2]; x_.w = PRE[4 * q + 3];                        \
     *(uint4*)(smem + (BUFI) * BUF + chunk_dst(q)) = x_;                  \
   }
-#define EDHMC_STEP(S, CUR, NXT)                                          \
+// One K stage: optionally load stage S + DEPTH into NXT, compute from LDS
+// buffer (S - s_begin) & 1, optionally store the stage held in CUR (S + 1)
+// into the other buffer, barrier.  In the steady-state loops the load is
+// never under a runtime branch: a load under a branch makes the compiler's
+// wait-count merge at the join assume the newest loads may be the ones the
+// store needs, i.e. wait for every outstanding load, which would turn the
+// prefetch into a stall per stage.  Only the peeled last stages branch.
+#define EDHMC_STEP(S, CUR, NXT, DEPTH, LOAD, STORE)                      \
   {                                                                      \
     const int buf_ = ((S) - s_begin) & 1;                                \
-    if ((S) + 2 < s_end) { EDHMC_GLOAD(NXT, (S) + 2) }                   \
+    if (LOAD) { EDHMC_GLOAD(NXT, (S) + (DEPTH)) }                        \
     compute(buf_);                                                       \
-    if ((S) + 1 < s_end) { EDHMC_SSTORE(CUR, buf_ ^ 1) }                 \
+    if (STORE) { EDHMC_SSTORE(CUR, buf_ ^ 1) }                           \
     __syncthreads();                                                     \
   }
-  if (s_begin < s_end) {
-    EDHMC_GLOAD(r0, s_begin)
-    EDHMC_SSTORE(r0, 0)
-    if (s_begin + 1 < s_end) { EDHMC_GLOAD(r1, s_begin + 1) }
+  if constexpr (kPrefetchDepth == 3) {
+    // invariant at the top of step s: stage s in LDS, stages s+1, s+2 in
+    // registers; the loop body rotates r1 (s+1), r2 (s+2), r0 (free)
+    if (s_begin < s_end) {
+      EDHMC_GLOAD(r0, s_begin)
+      EDHMC_SSTORE(r0, 0)
+      if (s_begin + 1 < s_end) { EDHMC_GLOAD(r1, s_begin + 1) }
+      if (s_begin + 2 < s_end) { EDHMC_GLOAD(r2, s_begin + 2) }
+    }
+    __syncthreads();
+    int s = s_begin;
+    for (; s + 5 < s_end; s += 3) {
+      EDHMC_STEP(s, r1, r0, 3, true, true)
+      EDHMC_STEP(s + 1, r2, r1, 3, true, true)
+      EDHMC_STEP(s + 2, r0, r2, 3, true, true)
+    }
+    // at most five stages left; the rotation is the loop's (s = s_begin mod 3)
+    if (s < s_end) EDHMC_STEP(s, r1, r0, 3, s + 3 < s_end, s + 1 < s_end)
+    if (s + 1 < s_end) EDHMC_STEP(s + 1, r2, r1, 3, s + 4 < s_end, s + 2 < s_end)
+    if (s + 2 < s_end) EDHMC_STEP(s + 2, r0, r2, 3, false, s + 3 < s_end)
+    if (s + 3 < s_end) EDHMC_STEP(s + 3, r1, r0, 3, false, s + 4 < s_end)
+    if (s + 4 < s_end) EDHMC_STEP(s + 4, r2, r1, 3, false, false)
+  } else {
+    // depth 2: stage s in LDS, stage s + 1 in r1 at the top of step s
+    if (s_begin < s_end) {
+      EDHMC_GLOAD(r0, s_begin)
+      EDHMC_SSTORE(r0, 0)
+      if (s_begin + 1 < s_end) { EDHMC_GLOAD(r1, s_begin + 1) }
+    }
+    __syncthreads();
+    int s = s_begin;
+    for (; s + 3 < s_end; s += 2) {
+      EDHMC_STEP(s, r1, r0, 2, true, true)
+      EDHMC_STEP(s + 1, r0, r1, 2, true, true)
+    }
+    if (s < s_end) EDHMC_STEP(s, r1, r0, 2, s + 2 < s_end, s + 1 < s_end)
+    if (s + 1 < s_end) EDHMC_STEP(s + 1, r0, r1, 2, false, s + 2 < s_end)
+    if (s + 2 < s_end) EDHMC_STEP(s + 2, r1, r0, 2, false, false)
   }
-  __syncthreads();
-  int s = s_begin;
-  for (; s + 1 < s_end; s += 2) {
-    EDHMC_STEP(s, r1, r0)
-    EDHMC_STEP(s + 1, r0, r1)
-  }
-  if (s < s_end) EDHMC_STEP(s, r1, r0)
 #undef EDHMC_STEP
 #undef EDHMC_SSTORE
 #undef EDHMC_GLOAD

@@ -26,7 +26,8 @@
 
 namespace edhmc {
 
-using CfgM = TileCfg<2, 2, 2, 2>;   // 128 x 128 tiles for every MLP GEMM
+using CfgM = TileCfg<2, 2, 2, 2>;    // 128 x 128 tiles
+using CfgMS = TileCfg<1, 1, 2, 2>;   // 64 x 64 tiles for GEMMs with few output tiles
 
 constexpr float kGeluK0 = 0.7978845608028654f;  // sqrt(2 / pi)
 constexpr float kGeluK1 = 0.044715f;
@@ -354,9 +355,9 @@ static PackLayout layout_of(const Surrogate* s) {
   return L;
 }
 
-template <typename Epi>
-static int mlp_gemm(edhmc_ctx* ctx, const void* A, int64_t lda_elems, const void* B,
-                    int64_t ldb_elems, int M, int N, int K, Epi e, hipStream_t st) {
+template <typename Cfg, typename Epi>
+static int mlp_gemm_cfg(edhmc_ctx* ctx, const void* A, int64_t lda_elems, const void* B,
+                        int64_t ldb_elems, int M, int N, int K, Epi e, hipStream_t st) {
   GemmShape g = {};
   g.A = (const char*)A;
   g.lda_b = lda_elems * 2;
@@ -365,16 +366,27 @@ static int mlp_gemm(edhmc_ctx* ctx, const void* A, int64_t lda_elems, const void
   g.B = (const char*)B;
   g.ldb_b = ldb_elems * 2;
   g.b_stage_b = kStageBytes;
-  g.m_tiles = M / CfgM::BM;
-  g.n_tiles = N / CfgM::BN;
+  g.m_tiles = M / Cfg::BM;
+  g.n_tiles = N / Cfg::BN;
   g.k_stages = K * 2 / kStageBytes;
   g.ksplit = 1;
   g.stages_per_split = g.k_stages;
   g.swz = 0;
+  if (!gemm_offsets_fit<Cfg, 1>(g)) return set_err(ctx, EDHMC_EUNSUPPORTED, "MLP tile too large");
   ProfScope ps(ctx, PK_MLP, st);
-  hipLaunchKernelGGL((gemm_nt_kernel<bf16_t, CfgM, 1, Epi>), dim3(grid_size(g)),
-                     dim3(CfgM::kThreads), 0, st, g, e);
+  hipLaunchKernelGGL((gemm_nt_kernel<bf16_t, Cfg, 1, Epi>), dim3(grid_size(g)),
+                     dim3(Cfg::kThreads), 0, st, g, e);
   return check_launch(ctx, "mlp_gemm");
+}
+
+// 128 x 128 tiles, or 64 x 64 when that leaves fewer than two workgroups per
+// CU (e.g. the output layer: 4096 x 512 is only 128 large tiles).
+template <typename Epi>
+static int mlp_gemm(edhmc_ctx* ctx, const void* A, int64_t lda_elems, const void* B,
+                    int64_t ldb_elems, int M, int N, int K, Epi e, hipStream_t st) {
+  if ((M / CfgM::BM) * (N / CfgM::BN) < 512)
+    return mlp_gemm_cfg<CfgMS>(ctx, A, lda_elems, B, ldb_elems, M, N, K, e, st);
+  return mlp_gemm_cfg<CfgM>(ctx, A, lda_elems, B, ldb_elems, M, N, K, e, st);
 }
 
 static int transpose(edhmc_ctx* ctx, const DevBuf& in, int R, int Cc, DevBuf& out, hipStream_t st) {

@@ -74,6 +74,8 @@ def ess(engine, samples, max_lag=64, between=True):
   """Per-coordinate ESS of samples [T, C, D] (device, fp32); the lag window
   doubles until every coordinate's Geyer sequence is truncated."""
   T = samples.shape[0]
+  if T < 4:
+    return np.full(int(samples.shape[2]), np.nan)   # too few draws to estimate
   lag = min(max_lag, T - 1)
   while True:
     acov, cmean = autocov(engine, samples, lag)

@@ -158,6 +158,22 @@ class GLMEngine(Context):
     self.check(self.lib.edhmc_logp(self.h, L.ptr(W), C, L.ptr(lp), self.stream))
     return lp
 
+  def predict(self, W, mean=True, loglik=True, replicate=False, seed=0):
+    """Posterior-predictive pass over the bound rows for draws W [S, D]:
+    (mean [N] of E[y | w_s] over the draws, loglik [S] fp64, y_rep [S, N]),
+    entries None where not asked for."""
+    W = _dev_tensor(W, torch.float32, self.device)
+    if W.dim() == 1:
+      W = W[None]
+    S = int(W.shape[0])
+    m = torch.empty(self.N, device=self.device) if mean else None
+    ll = torch.empty(S, dtype=torch.float64, device=self.device) if loglik else None
+    rep = torch.empty(S, self.N, device=self.device) if replicate else None
+    self.C = S
+    self.check(self.lib.edhmc_predict(self.h, L.ptr(W), S, L.ptr(m), L.ptr(ll), L.ptr(rep),
+                                      int(seed), self.stream))
+    return m, ll, rep
+
   # -- sampling ------------------------------------------------------------
   def transition(self, step_size, n_steps, momenta=None, uniforms=None, seed=0, counter=0,
                  sample_out=None, n_accept=None, accepted_out=None):

@@ -99,6 +99,23 @@ int edhmc_logp_grad(edhmc_ctx* ctx, const float* W, int32_t C, float* grad_out,
  * the current chain count discards the chain state. */
 int edhmc_logp(edhmc_ctx* ctx, const float* W, int32_t C, double* logp_out, void* stream);
 
+/* Posterior-predictive pass over the bound rows for S posterior draws
+ * W [S, D] (the sess.run loops over `output_key.probs` / `output_key` /
+ * `log_prob(y_true)` of edward/criticisms/evaluate.py:128-150,199-204 and the
+ * replicated data sets of edward/criticisms/ppc.py:100-121), any output may
+ * be NULL:
+ *   mean_out   [N] fp32: mean over the S draws of E[y_n | w_s] (sigmoid(eta)
+ *              for Bernoulli, eta for Normal)
+ *   loglik_out [S] fp64: sum over rows of log p(y_n | w_s), y = the bound y
+ *   rep_out    [S, N] fp32: y_rep ~ p(y | w_s), Philox4x32-10 keyed by seed,
+ *              counter (s, n) (Bernoulli: u < sigmoid(eta) with 24-bit u;
+ *              Normal: eta + s * Box-Muller normal)
+ * In bf16 precision the forward uses the hi/lo split of W.  Uses the chain
+ * buffers: a draw count different from the current chain count discards
+ * the chain state. */
+int edhmc_predict(edhmc_ctx* ctx, const float* W, int32_t S, float* mean_out,
+                  double* loglik_out, float* rep_out, uint64_t seed, void* stream);
+
 /* One HMC transition of every chain: momentum draw (hmc.py:88-91),
  * leapfrog with n_steps steps of size eps (hmc.py:94-97,195-210), ratio and
  * accept (hmc.py:100-111), selection, and the sample write + n_accept
