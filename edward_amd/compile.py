@@ -29,11 +29,16 @@ def _array(v):
   return np.asarray(v)
 
 
+class UnboundPlaceholder(ValueError):
+  """A placeholder the model needs has no value yet (it is fed later through
+  ``update(feed_dict=...)``)."""
+
+
 def _resolve(node, data):
   """Constant value of a data node (placeholder bound in data, or array)."""
   if isinstance(node, M.Placeholder):
     if node not in data:
-      raise ValueError('placeholder %s is not bound in data' % node.name)
+      raise UnboundPlaceholder('placeholder %s is not bound in data' % node.name)
     return _array(data[node]).astype(np.float64)
   if isinstance(node, (M.RandomVariable, M.Node)):
     return None
@@ -173,27 +178,69 @@ class GLMSpec(object):
     self.__dict__.update(kw)
 
 
-def compile_model(latent_vars, data, precision='fp32'):
-  spec = glm_spec(latent_vars, data)
+def compile_model(latent_vars, data, precision='fp32', scale=None):
+  """Bind the model to a device engine.  ``scale`` (Edward's per-variable
+  multipliers, inference.py:187-193) is honoured by the stochastic-gradient
+  samplers only; HMC passes None (hmc.py:161-192 ignores it)."""
+  spec = glm_spec(latent_vars, data, scale)
   if not isinstance(spec, GLMSpec):
     return spec          # another engine family built its own program
   from edward_amd.engine import GLMEngine
   eng = GLMEngine(spec.X, spec.y, spec.loc, spec.scale, likelihood=spec.likelihood,
                   lik_scale=spec.lik_scale, precision=precision)
+  if spec.lik_weight != 1.0:
+    eng.set_lik_weight(spec.lik_weight)
   return Program(eng, spec.latents, spec.n_rows, spec.n_chains, spec.dim)
 
 
-def glm_spec(latent_vars, data):
+def rebind_model(program, latent_vars, data, scale=None):
+  """Bind new data values (a fed minibatch) to a compiled program's engine;
+  the latent layout must not change.  Discards the engine's chain state."""
+  spec = glm_spec(latent_vars, data, scale)
+  if not isinstance(spec, GLMSpec):
+    raise NotImplementedError('feeding data is supported for the GLM family only')
+  if spec.dim != program.dim or spec.n_chains != program.n_chains:
+    raise ValueError('fed data changes the latent layout')
+  eng = program.engine
+  eng.bind(spec.X, spec.y, spec.loc, spec.scale, spec.likelihood, spec.lik_scale)
+  if spec.lik_weight != 1.0:
+    eng.set_lik_weight(spec.lik_weight)
+
+
+def _scalar_scale(scale, rv):
+  w = (scale or {}).get(rv, 1.0)
+  w = np.asarray(w.detach().cpu().numpy() if isinstance(w, torch.Tensor) else w, np.float64)
+  if w.size != 1:
+    raise NotImplementedError('only scalar scale factors are supported')
+  w = float(w.reshape(-1)[0])
+  if not w > 0:
+    raise ValueError('scale factors must be positive')
+  return w
+
+
+def _observed_values(obs, data):
+  v = data[obs]
+  if isinstance(v, M.Placeholder):
+    if v not in data:
+      raise UnboundPlaceholder('placeholder %s is not bound: feed it in '
+                               'update(feed_dict=...)' % v.name)
+    v = data[v]
+  return _array(v).astype(np.float64).reshape(-1)
+
+
+def glm_spec(latent_vars, data, scale=None):
   """Match the model to the GLM family; returns a GLMSpec (no device work)."""
   observed = [k for k in data if isinstance(k, M.RandomVariable)]
   if len(observed) != 1:
     raise NotImplementedError('exactly one observed random variable is supported, got %d'
                               % len(observed))
   obs = observed[0]
-  yv = _array(data[obs]).astype(np.float64).reshape(-1)
+  yv = _observed_values(obs, data)
   latents = list(latent_vars.keys())
   if isinstance(obs, M.Bernoulli) and isinstance(obs.logits, M.Add) and \
           _has_gather(obs.logits):
+    if scale:
+      raise NotImplementedError('scale is not supported for the hierarchical model')
     from edward_amd import hier
     return hier.compile_hier(latent_vars, data, obs, yv)
   if isinstance(obs, M.Bernoulli):
@@ -226,7 +273,9 @@ def glm_spec(latent_vars, data):
     if pr is None:
       raise NotImplementedError('prior parameters of %s must be constants' % rv.name)
     locs.append(pr[0])
-    scales.append(pr[1])
+    # a scale factor w on log N(z; m, s) has the gradient of N(z; m, s / sqrt(w))
+    # (the samplers that honour scale use gradients only)
+    scales.append(pr[1] / np.sqrt(_scalar_scale(scale, rv)))
     blk = np.zeros((N, k))
     for t in terms:
       if t.rv is rv:
@@ -242,7 +291,8 @@ def glm_spec(latent_vars, data):
   T = int(min(l.qz.params.shape[0] for l in lat))
   return GLMSpec(X=X, y=yv.astype(np.float32), loc=np.concatenate(locs),
                  scale=np.concatenate(scales), likelihood=likelihood, lik_scale=lik_scale,
-                 latents=lat, n_rows=T, n_chains=n_chains, dim=off)
+                 lik_weight=_scalar_scale(scale, obs), latents=lat, n_rows=T,
+                 n_chains=n_chains, dim=off)
 
 
 def _has_gather(E):

@@ -31,10 +31,12 @@ template <typename T> struct ElemOps;
 template <> struct ElemOps<float> {
   static constexpr int kBytes = 4;
   __device__ static inline float from_f32(float v) { return v; }
+  __device__ static inline float to_f32(float v) { return v; }
 };
 template <> struct ElemOps<bf16_t> {
   static constexpr int kBytes = 2;
   __device__ static inline bf16_t from_f32(float v) { return f32_to_bf16(v); }
+  __device__ static inline float to_f32(bf16_t v) { return bf16_to_f32(v); }
 };
 
 // ---------------------------------------------------------------------------
@@ -85,6 +87,17 @@ __device__ inline float log1p_unit(float e) {
   const float u = 1.0f + e;
   const float l = __logf(u);
   return u == 1.0f ? e : l * (e * fast_rcp(u - 1.0f));
+}
+
+// softplus(eta) = max(eta, 0) + log1p(exp(-|eta|)) with two native
+// transcendentals: t = 2^(-|eta| log2 e) in (0, 1]; log1p(t) as ln2 log2(1 + t),
+// or t (1 - t/2) once 1 + t would round t away (t < 2^-12: the dropped t^3/3
+// is below 2^-36).  Absolute error about 2^-24 per element.
+__device__ inline float softplus_fast(float eta) {
+  const float t = __builtin_amdgcn_exp2f(-fabsf(eta) * 1.4426950408889634f);
+  const float l = t < 2.44140625e-4f ? t * (1.0f - 0.5f * t)
+                                     : __builtin_amdgcn_logf(1.0f + t) * 0.6931471805599453f;
+  return fmaxf(eta, 0.0f) + l;
 }
 
 __device__ inline float glm_dll(const LikParams& p, float eta, float y) {

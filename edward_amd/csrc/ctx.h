@@ -69,6 +69,7 @@ struct edhmc_ctx {
   int cdt = EDHMC_F32;   // compute dtype of X / positions in the GEMMs
   int lik = EDHMC_LIK_BERNOULLI;
   float lik_scale = 1.0f;
+  float lik_weight = 1.0f;   // likelihood multiplier (Edward's `scale` for the SG samplers)
   int64_t N = 0, N_pad = 0;
   int D = 0, D_pad = 0;
   edhmc::DevBuf Xr, Xc, y, ploc, pscale;
@@ -95,6 +96,7 @@ struct edhmc_ctx {
   edhmc::DevBuf tmp_count, stats_part;
   edhmc::DevBuf pred_col;   // posterior-predictive column partials [m_tiles][N_pad]
   edhmc::DevBuf ll_split;   // log-likelihood range sums [16][C_pad] (fixed-order reduction)
+  edhmc::DevBuf xty, xty_part;   // X^T y (FP64, Bernoulli) and its row-chunk partials
 
   edhmc::Profiler prof;
   edhmc::Surrogate* sur = nullptr;
@@ -182,5 +184,6 @@ void hier_destroy(HierModel* h);
 // GLM pieces reused by the surrogate transition (edhmc_glm.hip).
 int glm_forward_ll(edhmc_ctx* ctx, hipStream_t st);   // llpart at z_lp (split)
 int glm_write_lp(edhmc_ctx* ctx, hipStream_t st);     // z -> z_lp planes
+int glm_refresh_logp(edhmc_ctx* ctx, hipStream_t st);
 
 }  // namespace edhmc

@@ -131,6 +131,11 @@ struct EpiGLM {
       gj[j] = (goff && valid[j]) ? gidx[n] : 0;
     }
     const int ml0 = p.m0 - p.tm * BM, nl0 = p.n0 - p.tn * BN;   // tile-local origin
+    // Forward-only Bernoulli pass: the linear part sum_n y_n eta_n = w . X^T y
+    // is added per chain in FP64 after the reduction (ll_linear_kernel), so the
+    // epilogue sums only -softplus(eta) (fp32 over the lane's WN columns, FP64
+    // across them).
+    const bool sp_only = LL && !WR && !goff && lp.lik == LIK_BERNOULLI;
 #pragma unroll
     for (int i = 0; i < WM; ++i)
 #pragma unroll
@@ -138,25 +143,32 @@ struct EpiGLM {
         const int mr = 32 * i + acc_row(r, p.lane);
         double llsum = 0.0;
         const float* grow = goff ? goff + (int64_t)(p.m0 + mr) * goff_ld : nullptr;
+        if (sp_only) {
+          float s32 = 0.0f;
 #pragma unroll
-        for (int j = 0; j < WN; ++j) {
-          const int nc = 32 * j + (p.lane & 31);
-          const float eta = acc[i][j][r] + (grow ? grow[gj[j]] : 0.0f);
-          float dll;
-          if (LL) {
-            double ll;
-            glm_both(lp, eta, yv[j], dll, ll);
-            llsum += valid[j] ? ll : 0.0;
-          } else {
-            dll = glm_dll(lp, eta, yv[j]);
-          }
-          if (WR) {
-            const T v = ElemOps<T>::from_f32(valid[j] ? dll : 0.0f);
-            if (stage) {
-              *(T*)(rt + (ml0 + mr) * RROW + (nl0 + nc) * (int)sizeof(T)) = v;
+          for (int j = 0; j < WN; ++j) s32 -= valid[j] ? softplus_fast(acc[i][j][r]) : 0.0f;
+          llsum = (double)s32;
+        } else {
+#pragma unroll
+          for (int j = 0; j < WN; ++j) {
+            const int nc = 32 * j + (p.lane & 31);
+            const float eta = acc[i][j][r] + (grow ? grow[gj[j]] : 0.0f);
+            float dll;
+            if (LL) {
+              double ll;
+              glm_both(lp, eta, yv[j], dll, ll);
+              llsum += valid[j] ? ll : 0.0;
             } else {
-              const int64_t m = p.m0 + mr, n = p.n0 + nc;
-              Rt[((n / KE) * ldr + m) * KE + (n % KE)] = v;
+              dll = glm_dll(lp, eta, yv[j]);
+            }
+            if (WR) {
+              const T v = ElemOps<T>::from_f32(valid[j] ? dll : 0.0f);
+              if (stage) {
+                *(T*)(rt + (ml0 + mr) * RROW + (nl0 + nc) * (int)sizeof(T)) = v;
+              } else {
+                const int64_t m = p.m0 + mr, n = p.n0 + nc;
+                Rt[((n / KE) * ldr + m) * KE + (n % KE)] = v;
+              }
             }
           }
         }
@@ -380,6 +392,55 @@ ll_final_kernel(const double* __restrict__ split, int C_pad, double* __restrict_
   ll[c] = s;
 }
 
+// X^T y in FP64 over the compute-precision rows (the values the GEMMs use):
+// pass 1 per (row chunk, column), pass 2 the chunks in a fixed order.
+template <typename T>
+__global__ void __launch_bounds__(256)
+xty_part_kernel(const T* __restrict__ Xr, const float* __restrict__ y, int64_t N, int D_pad,
+                int chunks, double* __restrict__ part) {
+  const int d = blockIdx.x * 256 + threadIdx.x;
+  if (d >= D_pad) return;
+  const int64_t per = (N + chunks - 1) / chunks;
+  const int64_t b = (int64_t)blockIdx.y * per, e = min(N, b + per);
+  double s = 0.0;
+  for (int64_t n = b; n < e; ++n)
+    s += (double)y[n] * (double)ElemOps<T>::to_f32(Xr[n * D_pad + d]);
+  part[(int64_t)blockIdx.y * D_pad + d] = s;
+}
+
+__global__ void __launch_bounds__(256)
+xty_final_kernel(const double* __restrict__ part, int chunks, int D_pad, double* __restrict__ xty) {
+  const int d = blockIdx.x * 256 + threadIdx.x;
+  if (d >= D_pad) return;
+  double s = 0.0;
+  for (int k = 0; k < chunks; ++k) s += part[(int64_t)k * D_pad + d];
+  xty[d] = s;
+}
+
+// ll[c] += w_c . X^T y (FP64), w_c read back from the GEMM's position planes
+// (hi + lo for bf16): completes the forward-only Bernoulli log-likelihood,
+// whose epilogue summed -softplus(eta) only.
+__global__ void __launch_bounds__(256)
+ll_linear_kernel(const void* __restrict__ z_lp, int bf16, int D, int D_pad, int C_pad,
+                 const double* __restrict__ xty, double* __restrict__ ll) {
+  __shared__ double red[4];
+  const int c = blockIdx.x;
+  double s = 0.0;
+  for (int d = threadIdx.x; d < D; d += 256) {
+    const int64_t idx = (int64_t)c * D_pad + d;
+    double w;
+    if (bf16) {
+      const bf16_t* p = (const bf16_t*)z_lp;
+      w = (double)bf16_to_f32(p[idx]) + (double)bf16_to_f32(p[idx + (int64_t)C_pad * D_pad]);
+    } else {
+      w = (double)((const float*)z_lp)[idx];
+    }
+    s += w * xty[d];
+  }
+  s = block_sum_f64<256>(s, red);
+  if (threadIdx.x == 0) ll[c] += s;
+}
+
 __global__ void pad_copy_kernel(const float* __restrict__ src, int64_t n, float* __restrict__ dst,
                                 int64_t n_pad, float fill, int* nonfinite) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -407,6 +468,7 @@ struct ChainArgs {
   uint64_t seed, counter;
   const float* Gpart; int ksplit;
   const double* ll;         // [C_pad] reduced log-likelihood
+  float lik_weight;         // multiplies the likelihood (scale of the SG samplers; 1 for HMC)
   float* sample_out; int32_t* accepted_out; int32_t* n_accept; int32_t* ndiv;
   double* mom_mean; double* mom_m2; int64_t mom_n;   // mom_n = count after update
   float* grad_out; double* logp_out;
@@ -481,7 +543,7 @@ __device__ inline float gsum(const ChainArgs& a, int c, int d) {
   const float* p = a.Gpart + (int64_t)c * a.D_pad + d;
   float s = 0.0f;
   for (int q = 0; q < a.ksplit; ++q) s += p[q * stride];
-  return s;
+  return a.lik_weight * s;
 }
 
 
@@ -530,7 +592,7 @@ __device__ inline void chain_accept(const ChainArgs& a, GradAt g_at) {
   }
   kl = block_sum_f64<kChainThreads>(kl, red);
   plp = block_sum_f64<kChainThreads>(plp, red);
-  const double ll = a.ll[c];
+  const double ll = (double)a.lik_weight * a.ll[c];
   const double nbad = block_sum_f64<kChainThreads>((double)bad, red);
   if (threadIdx.x == 0) {
     const double logpL = ll + plp;
@@ -588,7 +650,7 @@ __global__ void __launch_bounds__(kChainThreads) chain_eval_kernel(ChainArgs a) 
     plp += prior_logp(a, d, zv);
   }
   plp = block_sum_f64<kChainThreads>(plp, red);
-  const double ll = a.ll[c];
+  const double ll = (double)a.lik_weight * a.ll[c];
   if (threadIdx.x == 0) {
     a.logp[c] = ll + plp;
     if (a.logp_out) a.logp_out[c] = ll + plp;
@@ -629,7 +691,7 @@ __global__ void __launch_bounds__(kChainThreads) logp_out_kernel(ChainArgs a) {
   for (int d = threadIdx.x; d < a.D; d += kChainThreads)
     plp += prior_logp(a, d, a.W_in[(int64_t)c * a.D + d]);
   plp = block_sum_f64<kChainThreads>(plp, red);
-  if (threadIdx.x == 0) a.logp_out[c] = a.ll[c] + plp;
+  if (threadIdx.x == 0) a.logp_out[c] = (double)a.lik_weight * a.ll[c] + plp;
 }
 
 // Rewrite z_lp from z for every chain.
@@ -637,6 +699,54 @@ __global__ void __launch_bounds__(kChainThreads) chain_lp_kernel(ChainArgs a) {
   const int c = blockIdx.x;
   for (int d = threadIdx.x; d < a.D; d += kChainThreads)
     write_lp(a, c, d, a.z[(int64_t)c * a.D_pad + d]);
+}
+
+// Stochastic-gradient samplers: one workgroup per chain, the gradient at z in
+// g (scaled log joint), N(0, 1) draws from the momentum stream of (seed,
+// counter).  SGLD (edward/inferences/sgld.py:62-80):
+//   z <- z + lr/2 g + sqrt(lr) xi.
+// SGHMC (edward/inferences/sghmc.py:66-86), v a per-chain velocity:
+//   sample = z + v;  v <- (1 - friction/2) v + lr g + sqrt(lr friction) xi.
+// Both write the sample row and count it as accepted (n_accept += 1).
+template <bool HAMILTONIAN>
+__global__ void __launch_bounds__(kChainThreads)
+sg_step_kernel(ChainArgs a, float lr, float friction, float noise_sd, float* v) {
+  const int c = blockIdx.x;
+  const int nb = (a.D + 3) / 4;
+  for (int jb = threadIdx.x; jb < nb; jb += kChainThreads) {
+    const f32x4_t xi = momentum4(a.seed, a.counter, (uint32_t)jb, (uint32_t)c);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int d = 4 * jb + e;
+      if (d >= a.D) continue;
+      const int64_t idx = (int64_t)c * a.D_pad + d;
+      const float zv = a.z[idx], gv = a.g[idx];
+      float zn;
+      if (HAMILTONIAN) {
+        const int64_t vi = (int64_t)c * a.D + d;
+        const float vv = v[vi];
+        zn = zv + vv;
+        v[vi] = (1.0f - 0.5f * friction) * vv + lr * gv + noise_sd * xi[e];
+      } else {
+        zn = zv + 0.5f * lr * gv + noise_sd * xi[e];
+      }
+      a.z[idx] = zn;
+      write_lp(a, c, d, zn);
+      if (a.sample_out) a.sample_out[(int64_t)c * a.D + d] = zn;
+    }
+  }
+  if (threadIdx.x == 0 && a.n_accept) a.n_accept[c] += 1;
+}
+
+// logp[c] = ll[c] + prior log density at z[c] (after a forward-only pass).
+__global__ void __launch_bounds__(kChainThreads) chain_relogp_kernel(ChainArgs a) {
+  __shared__ double red[kChainThreads / 64];
+  const int c = blockIdx.x;
+  double plp = 0.0;
+  for (int d = threadIdx.x; d < a.D; d += kChainThreads)
+    plp += prior_logp(a, d, a.z[(int64_t)c * a.D_pad + d]);
+  plp = block_sum_f64<kChainThreads>(plp, red);
+  if (threadIdx.x == 0) a.logp[c] = (double)a.lik_weight * a.ll[c] + plp;
 }
 
 __global__ void copy_out_kernel(const float* __restrict__ src, int C, int D, int D_pad,
@@ -790,6 +900,7 @@ static ChainArgs chain_args(edhmc_ctx* ctx) {
   a.ploc = (const float*)ctx->ploc.p; a.pscale = (const float*)ctx->pscale.p;
   a.Gpart = (const float*)ctx->Gpart.p; a.ksplit = ctx->ksplit;
   a.ll = (const double*)ctx->ll.p;
+  a.lik_weight = ctx->lik_weight;
   a.ndiv = (int32_t*)ctx->ndiv.p;
   return a;
 }
@@ -912,8 +1023,24 @@ int glm_ensure_chains(edhmc_ctx* ctx, int C);
 
 // Forward only (log-likelihood partials, no residuals) at z_lp, exact form.
 int glm_forward_ll(edhmc_ctx* ctx, hipStream_t st) {
-  if (ctx->cdt == EDHMC_BF16) return launch_gemm1<bf16_t, CfgP2, 2, false, true>(ctx, st);
-  return launch_gemm1<float, Cfg1, 1, false, true>(ctx, st);
+  if (ctx->cdt == EDHMC_BF16) EDHMC_TRY((launch_gemm1<bf16_t, CfgP2, 2, false, true>(ctx, st)));
+  else EDHMC_TRY((launch_gemm1<float, Cfg1, 1, false, true>(ctx, st)));
+  if (ctx->lik != EDHMC_LIK_BERNOULLI) return EDHMC_OK;
+  hipLaunchKernelGGL(ll_linear_kernel, dim3(ctx->C), dim3(256), 0, st, (const void*)ctx->z_lp.p,
+                     ctx->cdt == EDHMC_BF16 ? 1 : 0, ctx->D, ctx->D_pad, ctx->C_pad,
+                     (const double*)ctx->xty.p, (double*)ctx->ll.p);
+  return check_launch(ctx, "ll_linear");
+}
+
+// Recompute the carried log joint at z with the forward-only pass (the
+// formula the surrogate's Metropolis test evaluates proposals with), so both
+// ends of its ratio use one formula after plain transitions.
+int glm_refresh_logp(edhmc_ctx* ctx, hipStream_t st) {
+  EDHMC_TRY(glm_write_lp(ctx, st));
+  EDHMC_TRY(glm_forward_ll(ctx, st));
+  ChainArgs a = chain_args(ctx);
+  hipLaunchKernelGGL(chain_relogp_kernel, dim3(ctx->C), dim3(kChainThreads), 0, st, a);
+  return check_launch(ctx, "chain_relogp");
 }
 
 int glm_write_lp(edhmc_ctx* ctx, hipStream_t st) {
@@ -1112,7 +1239,7 @@ edhmc_ctx::~edhmc_ctx() {
   if (hier) hier_destroy(hier);
   DevBuf* bufs[] = {&Xr, &Xc, &y, &ploc, &pscale, &z, &r, &g, &z0, &g0, &z_lp, &logp, &K0,
                     &flag, &ndiv, &Rt, &Gpart, &llpart, &ll, &mom_mean, &mom_m2, &tmp_count,
-                    &stats_part, &pred_col, &ll_split};
+                    &stats_part, &pred_col, &ll_split, &xty, &xty_part};
   for (DevBuf* b : bufs) b->release();
 }
 
@@ -1168,6 +1295,7 @@ int edhmc_bind_glm(edhmc_ctx* ctx, const void* X, int32_t x_in_dtype, const floa
   ctx->cdt = compute_dtype;
   ctx->lik = likelihood;
   ctx->lik_scale = lik_scale;
+  ctx->lik_weight = 1.0f;
   ctx->N = N;
   ctx->N_pad = round_up(N, kRowPad);
   ctx->D = D;
@@ -1214,6 +1342,23 @@ int edhmc_bind_glm(edhmc_ctx* ctx, const void* X, int32_t x_in_dtype, const floa
       hipStreamSynchronize(st) != hipSuccess)
     return set_err(ctx, EDHMC_EHIP, "bind: copy failed");
   if (nbad) return set_err(ctx, EDHMC_ENONFINITE, "data contains non-finite values (%d blocks)", nbad);
+  if (likelihood == EDHMC_LIK_BERNOULLI) {
+    // X^T y for the forward-only log-likelihood (its linear part)
+    const int chunks = (int)std::min<int64_t>(256, (N + 1023) / 1024);
+    EDHMC_TRY(dev_alloc(ctx, ctx->xty_part, (size_t)chunks * ctx->D_pad * 8, false));
+    EDHMC_TRY(dev_alloc(ctx, ctx->xty, (size_t)ctx->D_pad * 8, false));
+    const dim3 g1((ctx->D_pad + 255) / 256, chunks);
+    if (ctx->cdt == EDHMC_BF16)
+      hipLaunchKernelGGL(xty_part_kernel<bf16_t>, g1, dim3(256), 0, st, (const bf16_t*)ctx->Xr.p,
+                         (const float*)ctx->y.p, N, ctx->D_pad, chunks, (double*)ctx->xty_part.p);
+    else
+      hipLaunchKernelGGL(xty_part_kernel<float>, g1, dim3(256), 0, st, (const float*)ctx->Xr.p,
+                         (const float*)ctx->y.p, N, ctx->D_pad, chunks, (double*)ctx->xty_part.p);
+    EDHMC_TRY(check_launch(ctx, "xty_part"));
+    hipLaunchKernelGGL(xty_final_kernel, dim3((ctx->D_pad + 255) / 256), dim3(256), 0, st,
+                       (const double*)ctx->xty_part.p, chunks, ctx->D_pad, (double*)ctx->xty.p);
+    EDHMC_TRY(check_launch(ctx, "xty_final"));
+  }
   ctx->bound = true;
   return EDHMC_OK;
 }
@@ -1322,6 +1467,68 @@ int edhmc_transition(edhmc_ctx* ctx, float eps, int32_t n_steps, const float* r0
   DeviceGuard dg(ctx->device);
   return do_transition(ctx, eps, n_steps, r0, u, seed, counter, accepted_out, sample_out,
                        n_accept, (hipStream_t)stream);
+}
+
+static int sg_step(edhmc_ctx* ctx, bool hamiltonian, float lr, float friction, float* v,
+                   uint64_t seed, uint64_t counter, float* sample_out, int32_t* n_accept,
+                   hipStream_t st) {
+  if (!ctx->bound || !ctx->state_valid)
+    return set_err(ctx, EDHMC_ESTATE, "no chain state: call edhmc_set_state first");
+  if (!(lr >= 0.0f) || !std::isfinite(lr)) return set_err(ctx, EDHMC_EINVAL, "bad learning rate");
+  if (hamiltonian && (!v || !(friction >= 0.0f) || !std::isfinite(friction)))
+    return set_err(ctx, EDHMC_EINVAL, "SGHMC needs a velocity buffer and friction >= 0");
+  if (!ctx->g_valid) {
+    // gradient of the (scaled) log joint at the current positions
+    EDHMC_TRY(glm_write_lp(ctx, st));
+    EDHMC_TRY(grad_eval(ctx, true, true, st));
+    ChainArgs e = chain_args(ctx);
+    hipLaunchKernelGGL(chain_eval_kernel, dim3(ctx->C), dim3(kChainThreads), 0, st, e);
+    EDHMC_TRY(check_launch(ctx, "chain_eval"));
+  }
+  surrogate_invalidate(ctx->sur);
+  ChainArgs a = chain_args(ctx);
+  a.seed = seed;
+  a.counter = counter;
+  a.sample_out = sample_out;
+  a.n_accept = n_accept;
+  const float sd = hamiltonian ? sqrtf(lr * friction) : sqrtf(lr);
+  {
+    ProfScope ps(ctx, PK_CHAIN, st);
+    if (hamiltonian)
+      hipLaunchKernelGGL(sg_step_kernel<true>, dim3(ctx->C), dim3(kChainThreads), 0, st, a, lr,
+                         friction, sd, v);
+    else
+      hipLaunchKernelGGL(sg_step_kernel<false>, dim3(ctx->C), dim3(kChainThreads), 0, st, a, lr,
+                         0.0f, sd, (float*)nullptr);
+  }
+  EDHMC_TRY(check_launch(ctx, "sg_step"));
+  ctx->g_valid = false;   // positions moved; the next step re-evaluates the gradient
+  return EDHMC_OK;
+}
+
+int edhmc_set_lik_weight(edhmc_ctx* ctx, float w) {
+  if (!ctx) return EDHMC_EINVAL;
+  if (!(w > 0.0f) || !std::isfinite(w)) return set_err(ctx, EDHMC_EINVAL, "weight must be > 0");
+  ctx->lik_weight = w;
+  ctx->g_valid = false;
+  return EDHMC_OK;
+}
+
+int edhmc_sgld_step(edhmc_ctx* ctx, float learning_rate, uint64_t seed, uint64_t counter,
+                    float* sample_out, int32_t* n_accept, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  return sg_step(ctx, false, learning_rate, 0.0f, nullptr, seed, counter, sample_out, n_accept,
+                 (hipStream_t)stream);
+}
+
+int edhmc_sghmc_step(edhmc_ctx* ctx, float learning_rate, float friction, float* v,
+                     uint64_t seed, uint64_t counter, float* sample_out, int32_t* n_accept,
+                     void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  return sg_step(ctx, true, learning_rate, friction, v, seed, counter, sample_out, n_accept,
+                 (hipStream_t)stream);
 }
 
 int edhmc_run(edhmc_ctx* ctx, float eps, int32_t n_steps, float* params, int64_t T, int64_t t0,

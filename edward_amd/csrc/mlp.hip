@@ -698,6 +698,9 @@ static int sur_transition(edhmc_ctx* ctx, float eps, int L, const float* r0, con
   if (!std::isfinite(eps)) return set_err(ctx, EDHMC_EINVAL, "step_size must be finite");
   if (!s->norm_set) return set_err(ctx, EDHMC_ESTATE, "normalisation not set");
   if (!s->ghat_valid) EDHMC_TRY(eval_ghat(ctx, s, st));
+  // after plain transitions the carried log joint came from a gradient pass:
+  // re-evaluate it with the forward pass the accept test below uses
+  if (ctx->g_valid) EDHMC_TRY(glm_refresh_logp(ctx, st));
   const float half = (float)(0.5 * (double)eps), full = eps;
   const float* nrm = (const float*)s->nrm.p;
   SurChainArgs a;

@@ -90,6 +90,13 @@ class GLMEngine(Context):
   def __init__(self, X, y, prior_loc=0.0, prior_scale=1.0, likelihood='bernoulli',
                lik_scale=1.0, precision='fp32', device=0):
     super(GLMEngine, self).__init__(device)
+    self.precision = precision
+    self.C = None
+    self.bind(X, y, prior_loc, prior_scale, likelihood, lik_scale)
+
+  def bind(self, X, y, prior_loc=0.0, prior_scale=1.0, likelihood='bernoulli', lik_scale=1.0):
+    """(Re)bind the data and prior; discards the chain state (a new
+    minibatch of the stochastic-gradient samplers rebinds every update)."""
     dev = self.device
     if isinstance(X, torch.Tensor) and X.dtype == torch.bfloat16:
       Xd, xin = X.to(dev).contiguous(), L.BF16
@@ -108,15 +115,29 @@ class GLMEngine(Context):
         prior_scale.cpu() if isinstance(prior_scale, torch.Tensor) else prior_scale,
         np.float32), (D,)), torch.float32, dev)
     lik = {'bernoulli': L.LIK_BERNOULLI, 'normal': L.LIK_NORMAL}[likelihood]
-    cdt = {'fp32': L.F32, 'bf16': L.BF16}[precision]
+    cdt = {'fp32': L.F32, 'bf16': L.BF16}[self.precision]
     self.N, self.D = int(N), int(D)
-    self.precision = precision
     self.likelihood = likelihood
     self.C = None
     self.check(self.lib.edhmc_bind_glm(self.h, L.ptr(Xd), xin, L.ptr(yd), N, D, lik,
                                        float(lik_scale), L.ptr(loc), L.ptr(scale), cdt,
                                        self.stream))
     torch.cuda.synchronize(dev)
+
+  def set_lik_weight(self, w):
+    """Multiplier of the likelihood in the log joint (Edward's ``scale`` of
+    the observed variable, used by the stochastic-gradient samplers)."""
+    self.check(self.lib.edhmc_set_lik_weight(self.h, float(w)))
+
+  def sgld_step(self, learning_rate, seed=0, counter=0, sample_out=None, n_accept=None):
+    self.check(self.lib.edhmc_sgld_step(self.h, float(learning_rate), int(seed), int(counter),
+                                        L.ptr(sample_out), L.ptr(n_accept), self.stream))
+
+  def sghmc_step(self, learning_rate, friction, v, seed=0, counter=0, sample_out=None,
+                 n_accept=None):
+    self.check(self.lib.edhmc_sghmc_step(self.h, float(learning_rate), float(friction), L.ptr(v),
+                                         int(seed), int(counter), L.ptr(sample_out),
+                                         L.ptr(n_accept), self.stream))
 
   # -- state ---------------------------------------------------------------
   def set_state(self, W):

@@ -197,8 +197,68 @@ class MonteCarlo(Inference):
     self._reset_counts()
     self._state_t = None
 
+  # -- model binding (shared by the samplers) ------------------------------
+  _scale_aware = False     # SGLD / SGHMC honour `scale`; HMC ignores it (hmc.py:161-192)
+
+  def _prepare(self):
+    """Chain count and device from the stores; compile the model now if
+    every placeholder it needs is bound, else at the first fed update."""
+    from edward_amd.engine import require_gpu
+    require_gpu()
+    first = next(iter(self.latent_vars))
+    self.n_chains = _compile._chains_of(first, self.latent_vars[first])
+    self._device = torch.device('cuda', 0)
+    self.program = None
+    self.engine = None
+    self._state_t = None
+    try:
+      self._compile(self.data)
+    except _compile.UnboundPlaceholder:
+      pass
+
+  def _compile(self, data):
+    prog = _compile.compile_model(self.latent_vars, data, precision=self.precision,
+                                  scale=self.scale if self._scale_aware else None)
+    self.program = prog
+    self.engine = prog.engine
+    self.n_chains = prog.n_chains
+    self._device = self.engine.device
+    self._row = torch.empty(prog.n_chains, prog.dim, device=self._device)
+    self._on_compiled()
+
+  def _on_compiled(self):
+    pass
+
+  def _feed(self, feed_dict):
+    """Bind fed placeholder values (``monte_carlo.py:134-136``): the first
+    feed compiles the model, later feeds rebind the data on the device."""
+    data = dict(self.data)
+    data.update(feed_dict)
+    if self.program is None:
+      self._compile(data)
+    else:
+      _compile.rebind_model(self.program, self.latent_vars, data,
+                            self.scale if self._scale_aware else None)
+    self._state_t = None
+
+  def _check_bound(self):
+    if self.program is None:
+      raise ValueError('the model has unbound placeholders: pass them in '
+                       'update(feed_dict=...)')
+    T = self.program.n_rows
+    if self.t >= T:
+      raise IndexError('t=%d is past the %d rows of the Empirical store' % (self.t, T))
+
+  def _load_state(self):
+    need = max(self.t - 1, 0)
+    if self._state_t != need:
+      self.engine.set_state(self.program.gather(need))
+
   def update(self, feed_dict=None):
     """``monte_carlo.py:111-150``: one transition, then t += 1."""
+    if feed_dict:
+      self._feed(feed_dict)
+    self._check_bound()
     self._step()
     self.t += 1
     acc = self._n_accept.to(torch.float64).mean().item() / self.t
@@ -261,25 +321,16 @@ class HMC(MonteCarlo):
                        'unconstrained support. Please initialize with auto_transform=True '
                        'to ensure this. (if your variables already have unconstrained '
                        'support then doing this is a no-op).')
-    prog = _compile.compile_model(self.latent_vars, self.data, precision=self.precision)
-    self.program = prog
-    self.engine = prog.engine
-    self.n_chains = prog.n_chains
-    self._device = self.engine.device
-    self._state_t = None
-    self._row = torch.empty(prog.n_chains, prog.dim, device=self._device)
+    self._prepare()
+    return self
+
+  def _on_compiled(self):
     if self.surrogate is not None:
       self.surrogate.attach(self.engine)
-    return self
 
   def _step(self):
     t = self.t
-    T = self.program.n_rows
-    if t >= T:
-      raise IndexError('t=%d is past the %d rows of the Empirical store' % (t, T))
-    need = max(t - 1, 0)
-    if self._state_t != need:
-      self.engine.set_state(self.program.gather(need))
+    self._load_state()
     from edward_amd.engine import jittered_step
     eps = jittered_step(self.step_size, self.step_size_jitter, self.seed, t)
     if self.surrogate is None:
@@ -294,3 +345,71 @@ class HMC(MonteCarlo):
   @property
   def divergences(self):
     return self.engine.divergences().cpu().numpy()
+
+
+class _SGMonteCarlo(MonteCarlo):
+  """Shared set-up of the stochastic-gradient samplers: build-only keyword
+  arguments ``precision`` and ``seed`` as for HMC; ``scale`` multiplies the
+  observed variable's log likelihood and each latent's log prior (Edward's
+  minibatch scaling, ``inference.py:187-193``)."""
+
+  _scale_aware = True
+
+  def _common_initialize(self, kwargs):
+    self.precision = kwargs.pop('precision', 'fp32')
+    from edward_amd import util
+    seed = kwargs.pop('seed', None)
+    self.seed = util.get_seed() if seed is None else int(seed)
+
+  def build_update(self):
+    self._prepare()
+    return self
+
+
+class SGLD(_SGMonteCarlo):
+  """Stochastic gradient Langevin dynamics (``edward/inferences/sgld.py:10-121``,
+  Welling and Teh 2011): z_t = z_{t-1} + lr/2 grad log p(x, z_{t-1}) +
+  N(0, lr) with lr = step_size / (t + 1)^0.55; every draw is kept
+  (n_accept counts each update)."""
+
+  def initialize(self, step_size=0.25, *args, **kwargs):
+    """``sgld.py:45-54``; returns ``self``."""
+    self.step_size = step_size
+    self._common_initialize(kwargs)
+    return super(SGLD, self).initialize(*args, **kwargs)
+
+  def _step(self):
+    t = self.t
+    self._load_state()
+    lr = float(self.step_size) / float(t + 1) ** 0.55
+    self.engine.sgld_step(lr, seed=self.seed, counter=t, sample_out=self._row,
+                          n_accept=self._n_accept)
+    self.program.scatter(t, self._row)
+    self._state_t = t
+
+
+class SGHMC(_SGMonteCarlo):
+  """Stochastic gradient Hamiltonian Monte Carlo
+  (``edward/inferences/sghmc.py:11-135``, Chen et al. 2014 eq. 15) with a
+  velocity per chain that starts at zero (``sghmc.py:52-56``):
+  z_t = z_{t-1} + v, v <- (1 - friction/2) v + lr grad + N(0, lr friction),
+  lr = step_size * 0.01."""
+
+  def initialize(self, step_size=0.25, friction=0.1, *args, **kwargs):
+    """``sghmc.py:42-56``; returns ``self``."""
+    self.step_size = step_size
+    self.friction = friction
+    self._common_initialize(kwargs)
+    return super(SGHMC, self).initialize(*args, **kwargs)
+
+  def _on_compiled(self):
+    self.v = torch.zeros(self.program.n_chains, self.program.dim, device=self._device)
+
+  def _step(self):
+    t = self.t
+    self._load_state()
+    lr = float(self.step_size) * 0.01
+    self.engine.sghmc_step(lr, float(self.friction), self.v, seed=self.seed, counter=t,
+                           sample_out=self._row, n_accept=self._n_accept)
+    self.program.scatter(t, self._row)
+    self._state_t = t

@@ -134,6 +134,29 @@ int edhmc_transition(edhmc_ctx* ctx, float eps, int32_t n_steps, const float* r0
                      int32_t* accepted_out, float* sample_out, int32_t* n_accept,
                      void* stream);
 
+/* Stochastic-gradient samplers over the bound GLM (the chain state set by
+ * edhmc_set_state; the gradient at the current positions is evaluated first
+ * when the last call moved them).  Each call writes the new positions to
+ * sample_out [C, D] (may be NULL) and adds 1 to n_accept [C] (may be NULL),
+ * as edward/inferences/sgld.py:82-89 and sghmc.py:88-97 do.
+ *   edhmc_sgld_step  (sgld.py:62-80): z <- z + lr/2 grad + sqrt(lr) xi, with
+ *                    lr = step_size / (t + 1)^0.55 computed by the caller
+ *   edhmc_sghmc_step (sghmc.py:66-86): sample = z + v,
+ *                    v <- (1 - friction/2) v + lr grad + sqrt(lr friction) xi,
+ *                    lr = step_size * 0.01; v [C, D] fp32 in/out (caller owned,
+ *                    zero at the start as the reference's velocity variable)
+ * xi ~ N(0, I) from the momentum stream of (seed, counter). */
+int edhmc_sgld_step(edhmc_ctx* ctx, float learning_rate, uint64_t seed, uint64_t counter,
+                    float* sample_out, int32_t* n_accept, void* stream);
+int edhmc_sghmc_step(edhmc_ctx* ctx, float learning_rate, float friction, float* v,
+                     uint64_t seed, uint64_t counter, float* sample_out, int32_t* n_accept,
+                     void* stream);
+/* Multiplier of the likelihood term in the log joint and its gradient:
+ * Edward's `scale` of the observed variable (inference.py:187-193, used by
+ * sgld.py:95-121 and sghmc.py:99-135 for minibatches; HMC ignores it,
+ * hmc.py:161-192, so the HMC entry points see 1).  bind resets it to 1. */
+int edhmc_set_lik_weight(edhmc_ctx* ctx, float weight);
+
 /* The run loop of Inference.run (edward/inferences/inference.py:145-147):
  * transitions t = t0 .. t0+n_iter-1, each writing params[t] of the sample
  * store params [T, C, D] and using momentum counter counter0 + t.  The chain

@@ -423,6 +423,33 @@ def accept_uniforms(seed, counter, C):
 
 
 # ----------------------------------------------------------------------------
+# Stochastic-gradient samplers
+# ----------------------------------------------------------------------------
+
+
+def sgld_learning_rate(step_size, t):
+  """``sgld.py:66-67``: step_size / (t + 1)^0.55 (t = the update's index)."""
+  return step_size / float(t + 1) ** 0.55
+
+
+def sgld_step(z, grad, lr, xi):
+  """``sgld.py:70-80``: z + lr/2 grad + sqrt(lr) xi."""
+  return z + 0.5 * lr * grad + math.sqrt(lr) * xi
+
+
+def sghmc_step(z, v, grad, lr, friction, xi):
+  """``sghmc.py:73-86``: (z + v, (1 - friction/2) v + lr grad +
+  sqrt(lr friction) xi) with lr = step_size * 0.01 given."""
+  return z + v, (1.0 - 0.5 * friction) * v + lr * grad + math.sqrt(lr * friction) * xi
+
+
+def scaled_logp_grad(logp_grad_lik, prior_grad, lik_weight, prior_weight):
+  """Gradient of ``inference.py:187-193``'s scaled log joint: lik_weight times
+  the likelihood gradient plus prior_weight times the prior gradient."""
+  return lik_weight * logp_grad_lik + prior_weight * prior_grad
+
+
+# ----------------------------------------------------------------------------
 # Effective sample size
 # ----------------------------------------------------------------------------
 

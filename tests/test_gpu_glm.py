@@ -88,9 +88,9 @@ def test_logp_grad_bf16(N, D, C):
 
 @pytest.mark.parametrize('precision', ['fp32', 'bf16'])
 def test_forward_only_logp(precision):
-  """edhmc_logp (no gradient, the Metropolis test's forward) equals the log
-  joint of edhmc_logp_grad and the oracle; fp32 bar as above, bf16 rtol 1e-5
-  (X rounded to bf16 on both sides)."""
+  """edhmc_logp (no gradient, the Metropolis test's forward) against the
+  oracle (fp32 rtol 1e-6, bf16 rtol 1e-5 with X rounded to bf16 on both
+  sides) and against the log joint of edhmc_logp_grad."""
   from edward_amd import workloads as W
   wl = W.logreg(3000, 70, 150, 0.01, 10, 10, seed=15, bf16=(precision == 'bf16'))
   eng = _engine(wl, precision=precision)
@@ -98,8 +98,10 @@ def test_forward_only_logp(precision):
   lp = eng.logp(W0).cpu().numpy()
   lp_ref, _ = _oracle(wl).logp_grad(W0.astype(np.float64))
   np.testing.assert_allclose(lp, lp_ref, rtol=1e-5 if precision == 'bf16' else 1e-6)
+  # the gradient pass sums y eta - softplus(eta) per element; the forward pass
+  # sums -softplus(eta) and adds w . X^T y in FP64: both within the bar above
   lp2, _ = eng.logp_grad(W0)
-  np.testing.assert_allclose(lp, lp2.cpu().numpy(), rtol=1e-9)
+  np.testing.assert_allclose(lp, lp2.cpu().numpy(), rtol=2e-6)
 
 
 def test_nonfinite_data_and_positions_raise():

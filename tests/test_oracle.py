@@ -359,3 +359,39 @@ def test_laplace_matches_long_hmc_on_small_logreg():
   se = np.sqrt(np.diag(cov) / (s.shape[0] / 10))
   assert np.all(np.abs(s.mean(0) - w) < 5 * se)
   np.testing.assert_allclose(s.var(0), np.diag(cov), rtol=0.15)
+
+
+# --------------------------------------------------------------------------
+# Stochastic-gradient samplers
+# --------------------------------------------------------------------------
+
+def test_sgld_gaussian_stationary_variance_closed_form():
+  """Constant-rate SGLD on N(0, s^2) is the linear recursion
+  z' = (1 - lr/(2 s^2)) z + sqrt(lr) xi with stationary variance
+  s^2 / (1 - lr/(4 s^2)); the oracle step must reproduce it."""
+  rng = np.random.default_rng(3)
+  s2, lr, C = 2.0, 0.4, 200000
+  z = np.zeros(C)
+  for _ in range(200):
+    z = O.sgld_step(z, -z / s2, lr, rng.normal(size=C))
+  want = s2 / (1.0 - lr / (4.0 * s2))
+  assert abs(z.var() / want - 1.0) < 0.02
+  assert O.sgld_learning_rate(0.25, 0) == 0.25
+  assert O.sgld_learning_rate(1.0, 3) == pytest.approx(4.0 ** -0.55)
+
+
+def test_sghmc_gaussian_stationary_moments():
+  """SGHMC with friction f on N(0, s^2): (z, v) is a linear AR(1) system;
+  its stationary covariance solves the discrete Lyapunov equation."""
+  from scipy import linalg
+  rng = np.random.default_rng(4)
+  s2, lr, f, C = 1.5, 0.05, 0.3, 100000
+  z, v = np.zeros(C), np.zeros(C)
+  for _ in range(2000):
+    z, v = O.sghmc_step(z, v, -z / s2, lr, f, rng.normal(size=C))
+  # state (z, v): z' = z + v ; v' = -lr/s2 z + (1 - f/2) v + sqrt(lr f) xi
+  A = np.array([[1.0, 1.0], [-lr / s2, 1.0 - 0.5 * f]])
+  Q = np.array([[0.0, 0.0], [0.0, lr * f]])
+  P = linalg.solve_discrete_lyapunov(A, Q)
+  assert abs(z.var() / P[0, 0] - 1.0) < 0.03
+  assert abs(v.var() / P[1, 1] - 1.0) < 0.03
