@@ -10,9 +10,10 @@ Flow (one JSON line on stdout at the end; progress on stderr):
      chains are reset to their N(0, I) start;
   timed region (everything the surrogate method costs):
   1. burn-in: plain bf16 HMC from N(0, I), step size jittered log-uniformly
-     over [eps/2, 2 eps], until the mean log joint stops moving (at most
-     --burn-in transitions); then 8 transitions record (w, exact gradient)
-     at every 4th inner leapfrog point (~230k pairs);
+     around a centre that follows the acceptance rate, until the mean log
+     joint stops moving (at most --burn-in transitions); the last 8
+     transitions (once the mean moves by < 300 over 5 transitions) record
+     (w, exact gradient) at every 4th inner leapfrog point (~230k pairs);
   2. surrogate training on the GPU (own fwd/bwd/Adam kernels, early stopping
      on 5% held-out pairs);
   3. K surrogate transitions (exact bf16/FP64 log joint in every Metropolis
@@ -27,6 +28,7 @@ measured with hipEvents recorded by the library on its launch stream.
 """
 
 import argparse
+import math
 import json
 import os
 import sys
@@ -58,6 +60,11 @@ def parse():
                   help='untimed plain bf16 transitions (kernel load, first touch)')
   ap.add_argument('--burn-in', type=int, default=40,
                   help='max plain bf16 burn-in transitions (timed, part of the method)')
+  ap.add_argument('--rec-start-tol', type=float, default=300.0,
+                  help='pair recording starts when the mean log joint moves less than this '
+                       'over 5 burn-in transitions')
+  ap.add_argument('--burn-in-adapt', type=int, default=1,
+                  help='1: the burn-in step-size centre follows the acceptance rate')
   ap.add_argument('--plain-steps', type=int, default=8, help='timed plain fp32 transitions')
   ap.add_argument('--workload', default='large',
                   choices=['large', 'mid', 'small', 'hier', 'copy'],
@@ -238,30 +245,48 @@ def main():
   t = 0
   converged_at = None
   jit = np.random.default_rng(args.seed + 7)
-  while t < args.burn_in:
-    # the step size is jittered log-uniformly over [eps/2, 2 eps] during
-    # warm-up: with a fixed eps L, posterior directions whose frequency makes
-    # eps L a multiple of half a period keep their energy through every
-    # momentum refresh and approach the typical set very slowly
-    eps_t = eps * 2.0 ** jit.uniform(-1.0, 1.0)
+  log_c = math.log(eps)
+  recording, n_recorded = False, 0
+  while t < args.burn_in + n_rec:
+    # Pair recording overlaps the end of the burn-in: it starts once the mean
+    # log joint moves by less than --rec-start-tol over 5 transitions (near
+    # equilibrium) and records n_rec transitions; the burn-in ends when the
+    # mean moves by less than 20 over 5 transitions and the recording is full.
+    if not recording and t >= 10 and abs(means[-1] - means[-6]) < args.rec_start_tol:
+      bf.record_pairs(Wrec, Grec, stride=rec_stride)
+      recording = True
+    # the step size is jittered log-uniformly over [c/2, 2c] during burn-in:
+    # with a fixed eps L, posterior directions whose frequency makes eps L a
+    # multiple of half a period keep their energy through every momentum
+    # refresh and approach the typical set very slowly.  With --burn-in-adapt
+    # the centre c follows the acceptance rate (up while the chains still
+    # fall toward the mode and accept everything, down near equilibrium),
+    # within [eps/4, 4 eps]; the kept samples use eps itself.  Recording
+    # transitions use the sampling step size.
+    if recording:
+      eps_t = jittered_step(eps, args.jitter, args.seed, t)
+    else:
+      eps_t = math.exp(log_c) * 2.0 ** jit.uniform(-1.0, 1.0)
     n_acc.zero_()
     bf.transition(eps_t, L, seed=args.seed, counter=t, n_accept=n_acc)
     _, lp, _ = bf.get_state()
     means.append(lp.mean().item())
-    log('warm-up %d eps %.2e mean logp %.2f accept %.3f'
-        % (t, eps_t, means[-1], n_acc.float().mean().item()))
+    acc_t = n_acc.float().mean().item()
+    log('warm-up %d eps %.2e mean logp %.2f accept %.3f%s'
+        % (t, eps_t, means[-1], acc_t, ' (recording)' if recording else ''))
+    if args.burn_in_adapt and not recording:
+      log_c = min(max(log_c + 0.6 * (acc_t - 0.7), math.log(eps / 4)), math.log(4 * eps))
     t += 1
+    if recording:
+      n_recorded += 1
     if converged_at is None and t >= 10 and abs(means[-1] - means[-6]) < 20.0:
       converged_at = t
+    if converged_at is not None and n_recorded >= n_rec:
       break
-  bf.record_pairs(Wrec, Grec, stride=rec_stride)
-  for k in range(n_rec):
-    bf.transition(jittered_step(eps, args.jitter, args.seed, t), L, seed=args.seed, counter=t,
-                  n_accept=n_acc)
-    t += 1
   bf.record_pairs(None)
   torch.cuda.synchronize()
   n_pairs = bf.record_count()
+  n_warm = t
   warm_s = time.time() - t_warm0
   log('warm-up done: %d transitions, %d pairs, %.1f s' % (t, n_pairs, warm_s))
 
@@ -375,9 +400,9 @@ def main():
       'steps': K,
       'warmup': args.warmup,
       'ms_per_step': incl_s / K * 1e3,
-      'timed_region': 'burn-in (%d plain bf16 transitions) + pair recording (%d) + surrogate '
-                      'training + %d surrogate transitions: value = min-ESS of the %d samples '
-                      '/ all of it' % (t - K - n_rec, n_rec, K, K),
+      'timed_region': 'burn-in (%d plain bf16 transitions, pairs recorded over the last %d) + '
+                      'surrogate training + %d surrogate transitions: value = min-ESS of the %d '
+                      'samples / all of it' % (n_warm, n_rec, K, K),
       'higher_is_better': True,
       'scaling': 'weak',
       'vs_baseline': None,

@@ -63,7 +63,7 @@ int prof_collect(edhmc_ctx* ctx) {
 #define EDHMC_CFG_G2 4, 4, 2, 2
 #endif
 #ifndef EDHMC_CFG_P2
-#define EDHMC_CFG_P2 1, 4, 2, 2
+#define EDHMC_CFG_P2 1, 4, 2, 2, 64
 #endif
 #ifndef EDHMC_GEMM1_SWZ
 #define EDHMC_GEMM1_SWZ 3
@@ -922,14 +922,14 @@ static int launch_gemm1(edhmc_ctx* ctx, hipStream_t st, const float* goff = null
   GemmShape g = {};
   g.A = (const char*)ctx->z_lp.p;
   g.lda_b = (int64_t)ctx->D_pad * eb;
-  g.a_stage_b = kStageBytes;
+  g.a_stage_b = Cfg::KS;
   g.a_plane_b = (int64_t)ctx->C_pad * ctx->D_pad * eb;
   g.B = (const char*)ctx->Xr.p;
   g.ldb_b = (int64_t)ctx->D_pad * eb;
-  g.b_stage_b = kStageBytes;
+  g.b_stage_b = Cfg::KS;
   g.m_tiles = ctx->C_pad / Cfg::BM;
   g.n_tiles = (int)(ctx->N_pad / Cfg::BN);
-  g.k_stages = ctx->D_pad * eb / kStageBytes;
+  g.k_stages = ctx->D_pad * eb / Cfg::KS;
   g.ksplit = 1;
   g.stages_per_split = g.k_stages;
   g.swz = (kGemm1Swz == 3 && g.m_tiles % 8 == 0) ? 3 : 1;
@@ -1059,14 +1059,14 @@ static int launch_predict(edhmc_ctx* ctx, int S, bool want_mean, float* rep, uin
   GemmShape g = {};
   g.A = (const char*)ctx->z_lp.p;
   g.lda_b = (int64_t)ctx->D_pad * eb;
-  g.a_stage_b = kStageBytes;
+  g.a_stage_b = Cfg::KS;
   g.a_plane_b = (int64_t)ctx->C_pad * ctx->D_pad * eb;
   g.B = (const char*)ctx->Xr.p;
   g.ldb_b = (int64_t)ctx->D_pad * eb;
-  g.b_stage_b = kStageBytes;
+  g.b_stage_b = Cfg::KS;
   g.m_tiles = ctx->C_pad / Cfg::BM;
   g.n_tiles = (int)(ctx->N_pad / Cfg::BN);
-  g.k_stages = ctx->D_pad * eb / kStageBytes;
+  g.k_stages = ctx->D_pad * eb / Cfg::KS;
   g.ksplit = 1;
   g.stages_per_split = g.k_stages;
   g.swz = 1;
@@ -1299,7 +1299,7 @@ int edhmc_bind_glm(edhmc_ctx* ctx, const void* X, int32_t x_in_dtype, const floa
   ctx->N = N;
   ctx->N_pad = round_up(N, kRowPad);
   ctx->D = D;
-  ctx->D_pad = (int)round_up(D, 32);
+  ctx->D_pad = (int)round_up(D, 64);   // K of GEMM1 in whole 128-byte stages
   const int eb = elem_bytes(ctx);
   EDHMC_TRY(dev_alloc(ctx, ctx->Xr, (size_t)ctx->N_pad * ctx->D_pad * eb, false));
   EDHMC_TRY(dev_alloc(ctx, ctx->Xc, (size_t)ctx->N_pad * ctx->D_pad * eb, false));

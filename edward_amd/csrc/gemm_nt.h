@@ -144,12 +144,18 @@ __device__ inline int acc_row(int r, int lane) {
   return ((r >> 2) << 3) + ((lane >> 5) << 2) + (r & 3);
 }
 
-template <int WM, int WN, int WAVES_M, int WAVES_N>
+// WM x WN blocks of 32 x 32 per wave, WAVES_M x WAVES_N waves, KSB bytes of
+// K per pipeline stage (64 or 128; the operand rows are staged in LDS at
+// KSB + 16 bytes per row).
+template <int WM, int WN, int WAVES_M, int WAVES_N, int KSB = kStageBytes>
 struct TileCfg {
   static constexpr int kWM = WM, kWN = WN, kWavesM = WAVES_M, kWavesN = WAVES_N;
   static constexpr int kThreads = 64 * WAVES_M * WAVES_N;
   static constexpr int BM = 32 * WM * WAVES_M;
   static constexpr int BN = 32 * WN * WAVES_N;
+  static constexpr int KS = KSB;
+  static constexpr int RS = KSB + (kRowStride - kStageBytes);
+  static_assert(KSB % 64 == 0, "stage bytes");
 };
 
 // Epilogue context handed to the epilogue functor.
@@ -168,10 +174,10 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
   constexpr int BM = Cfg::BM, BN = Cfg::BN;
   constexpr int AR = BM * A_PLANES;
   constexpr int ROWS = AR + BN;
-  constexpr int CPR = kStageBytes / 16;          // 16-byte chunks per row
+  constexpr int CPR = Cfg::KS / 16;              // 16-byte chunks per row
   constexpr int CHUNKS = ROWS * CPR;
   constexpr int PER_T = (CHUNKS + NT - 1) / NT;
-  constexpr int BUF = ROWS * kRowStride;
+  constexpr int BUF = ROWS * Cfg::RS;
   __shared__ __align__(16) char smem[2 * BUF];
 
   int tm, tn, ks;
@@ -250,7 +256,7 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
     int c = tid + q * NT;
     if (c >= CHUNKS) c = CHUNKS - 1;
     const int row = c / CPR, col = c - row * CPR;
-    return row * kRowStride + col * 16;
+    return row * Cfg::RS + col * 16;
   };
 
   f32x16_t acc[WM][WN];
@@ -265,17 +271,17 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
   auto compute = [&](int buf) {
     const char* base = smem + buf * BUF;
 #pragma unroll
-    for (int kc = 0; kc < kStageBytes / 32; ++kc) {
+    for (int kc = 0; kc < Cfg::KS / 32; ++kc) {
       uint4 a[A_PLANES][WM], b[WN];
 #pragma unroll
       for (int p = 0; p < A_PLANES; ++p)
 #pragma unroll
         for (int i = 0; i < WM; ++i)
-          a[p][i] = *(const uint4*)(base + (p * BM + wm * 32 * WM + 32 * i + lrow) * kRowStride +
+          a[p][i] = *(const uint4*)(base + (p * BM + wm * 32 * WM + 32 * i + lrow) * Cfg::RS +
                                     kc * 32 + lhalf);
 #pragma unroll
       for (int j = 0; j < WN; ++j)
-        b[j] = *(const uint4*)(base + (AR + wn * 32 * WN + 32 * j + lrow) * kRowStride +
+        b[j] = *(const uint4*)(base + (AR + wn * 32 * WN + 32 * j + lrow) * Cfg::RS +
                                kc * 32 + lhalf);
 #pragma unroll
       for (int i = 0; i < WM; ++i)
