@@ -67,11 +67,12 @@ def parse():
                   help='1: the burn-in step-size centre follows the acceptance rate')
   ap.add_argument('--plain-steps', type=int, default=8, help='timed plain fp32 transitions')
   ap.add_argument('--workload', default='large',
-                  choices=['large', 'mid', 'small', 'hier', 'copy'],
+                  choices=['large', 'mid', 'small', 'hier', 'copy', 'forward'],
                   help="'hier': the hierarchical logistic regression (G=2048 groups of "
                        "1+Poisson(400) rows, D=64, 512 chains, L=16), its own JSON line; "
                        "'copy': 1 GiB device-to-device copies, the byte-counter calibration "
-                       "of HBM traffic profiles")
+                       "of HBM traffic profiles; 'forward': --steps exact forward passes "
+                       "(the gemm_ll kernel alone) on the large workload, for profiling")
   ap.add_argument('--hidden', type=int, default=1024)
   ap.add_argument('--epochs', type=int, default=12)
   ap.add_argument('--lr', type=float, default=3e-3)
@@ -182,6 +183,20 @@ def bench_hier(args):
   print(json.dumps(out))
 
 
+def measured_traffic(kernel, workload):
+  """HBM bytes per launch of `kernel` from the committed rocprofv3 byte
+  counters (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE per dispatch,
+  each scaled by its reading on a copy of known size), or None."""
+  path = os.path.join(ROOT, 'profiles', 'traffic.json')
+  if workload != 'large' or not os.path.exists(path):
+    return None
+  rec = json.load(open(path)).get(kernel)
+  if not rec:
+    return None
+  return 1024.0 * (rec['fetch_kib_per_dispatch'] * rec['fetch_scale'] +
+                   rec['write_kib_per_dispatch'] * rec['write_scale'])
+
+
 def bench_copy(args):
   """Copies of a 1 GiB buffer: a kernel of known traffic (1 GiB read and
   1 GiB written per launch) against which the profiler's byte counters are
@@ -201,6 +216,34 @@ def bench_copy(args):
                     'bytes_written_per_launch': 4 * n, 'launches': args.steps}))
 
 
+def bench_forward(args):
+  """--steps exact forward log-joint evaluations (gemm_ll) of 4096 chains on
+  the large logistic-regression workload: the dominant kernel of the
+  surrogate transition in isolation, for counter profiles."""
+  import torch
+  from edward_amd import workloads as W
+  from edward_amd.engine import GLMEngine
+  wl = W.logreg_large()
+  rng = np.random.default_rng(args.seed)
+  Wp = torch.tensor((wl.w_true[None, :] + 0.003 * rng.normal(size=(wl.C, wl.D)))
+                    .astype(np.float32), device='cuda')
+  eng = GLMEngine(wl.X, wl.y, wl.prior_loc, wl.prior_scale, precision='bf16')
+  eng.logp(Wp)
+  torch.cuda.synchronize()
+  eng.profile(True)
+  t0 = time.time()
+  for _ in range(args.steps):
+    eng.logp(Wp)
+  torch.cuda.synchronize()
+  secs = time.time() - t0
+  ms, n = eng.profile_read()['gemm_ll']
+  flop = 2.0 * wl.N * wl.D * wl.C
+  print(json.dumps({'metric': 'exact forward log joint, N=%d D=%d chains=%d' % (wl.N, wl.D, wl.C),
+                    'value': args.steps / secs, 'unit': 'evaluations/s', 'steps': args.steps,
+                    'gemm_ll_ms': ms / max(n, 1),
+                    'achieved_tflops': flop / (ms / max(n, 1) * 1e-3) / 1e12}))
+
+
 def main():
   args = parse()
   if args.gpus != 1:
@@ -212,6 +255,8 @@ def main():
     return bench_hier(args)
   if args.workload == 'copy':
     return bench_copy(args)
+  if args.workload == 'forward':
+    return bench_forward(args)
   import torch
   from edward_amd import diagnostics
   from edward_amd.engine import GLMEngine
@@ -250,9 +295,12 @@ def main():
   while t < args.burn_in + n_rec:
     # Pair recording overlaps the end of the burn-in: it starts once the mean
     # log joint moves by less than --rec-start-tol over 5 transitions (near
-    # equilibrium) and records n_rec transitions; the burn-in ends when the
-    # mean moves by less than 20 over 5 transitions and the recording is full.
-    if not recording and t >= 10 and abs(means[-1] - means[-6]) < args.rec_start_tol:
+    # equilibrium), or after --burn-in transitions, and records n_rec
+    # transitions; the burn-in ends when the mean moves by less than 20 over 5
+    # transitions (or --burn-in + n_rec transitions ran) and the recording is
+    # full.
+    if not recording and (t >= args.burn_in or
+                          (t >= 10 and abs(means[-1] - means[-6]) < args.rec_start_tol)):
       bf.record_pairs(Wrec, Grec, stride=rec_stride)
       recording = True
     # the step size is jittered log-uniformly over [c/2, 2c] during burn-in:
@@ -281,7 +329,7 @@ def main():
       n_recorded += 1
     if converged_at is None and t >= 10 and abs(means[-1] - means[-6]) < 20.0:
       converged_at = t
-    if converged_at is not None and n_recorded >= n_rec:
+    if n_recorded >= n_rec and (converged_at is not None or t >= args.burn_in + n_rec):
       break
   bf.record_pairs(None)
   torch.cuda.synchronize()
@@ -383,7 +431,8 @@ def main():
   achieved = flop / (ms / cnt * 1e-3)
   roof = {'bound': 'mfma', 'kernel': dom, 'achieved': achieved / 1e12,
           'peak': PEAK['bf16'] / 1e12, 'unit': 'TFLOP/s', 'frac': achieved / PEAK['bf16'],
-          'traffic': None, 'avg_launch_ms': ms / cnt, 'flop_per_launch': flop,
+          'traffic': measured_traffic(dom, args.workload), 'avg_launch_ms': ms / cnt,
+          'flop_per_launch': flop,
           'mfma_issued_frac': None if issued is None else issued / (ms / cnt * 1e-3) / PEAK['bf16'],
           'per_transition_ms': {k: v[0] / K for k, v in kinds.items()}}
 

@@ -109,6 +109,10 @@ inline bool gemm_offsets_fit(const GemmShape& g) {
 #define EDHMC_PREFETCH_DEPTH 3
 #endif
 constexpr int kPrefetchDepth = EDHMC_PREFETCH_DEPTH;   // register stages in flight (2 or 3)
+#ifndef EDHMC_MFMA_PRIO
+#define EDHMC_MFMA_PRIO 0
+#endif
+constexpr bool kMfmaPrio = EDHMC_MFMA_PRIO != 0;
 static_assert(kPrefetchDepth == 2 || kPrefetchDepth == 3, "prefetch depth");
 
 // Word 3 of a raw (unswizzled, stride 0) buffer resource on gfx9-family
@@ -270,6 +274,8 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
   const int lrow = lane & 31, lhalf = (lane >> 5) * 16;
   auto compute = [&](int buf) {
     const char* base = smem + buf * BUF;
+    // a wave in its MFMA phase issues ahead of waves loading or storing
+    if (kMfmaPrio) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int kc = 0; kc < Cfg::KS / 32; ++kc) {
       uint4 a[A_PLANES][WM], b[WN];
@@ -290,6 +296,7 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
 #pragma unroll
           for (int p = 0; p < A_PLANES; ++p) Mfma<T>::chunk(acc[i][j], a[p][i], b[j]);
     }
+    if (kMfmaPrio) __builtin_amdgcn_s_setprio(0);
   };
 
   // Register prefetch sets: each is one vector-register tuple (ext_vector of
