@@ -749,6 +749,69 @@ __global__ void __launch_bounds__(kChainThreads) chain_relogp_kernel(ChainArgs a
   if (threadIdx.x == 0) a.logp[c] = (double)a.lik_weight * a.ll[c] + plp;
 }
 
+// Random-walk Metropolis-Hastings (edward/inferences/metropolis_hastings.py:
+// 64-156) with the proposal Normal(loc = z, scale = s): z0 <- z,
+// z <- z + s xi (xi from the momentum stream), z_lp for the forward pass.
+__global__ void __launch_bounds__(kChainThreads)
+mh_propose_kernel(ChainArgs a, const float* __restrict__ scale) {
+  const int c = blockIdx.x;
+  const int nb = (a.D + 3) / 4;
+  for (int jb = threadIdx.x; jb < nb; jb += kChainThreads) {
+    const f32x4_t xi = momentum4(a.seed, a.counter, (uint32_t)jb, (uint32_t)c);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int d = 4 * jb + e;
+      if (d >= a.D) continue;
+      const int64_t idx = (int64_t)c * a.D_pad + d;
+      const float zv = a.z[idx];
+      const float zn = zv + scale[d] * xi[e];
+      a.z0[idx] = zv;
+      a.z[idx] = zn;
+      write_lp(a, c, d, zn);
+    }
+  }
+}
+
+// The symmetric proposal's log densities cancel (metropolis_hastings.py:
+// 101-115), so ratio = log p(x, z') - log p(x, z); accept iff log u < ratio
+// and the proposal is finite (a non-finite one is rejected and counted).
+__global__ void __launch_bounds__(kChainThreads) mh_accept_kernel(ChainArgs a) {
+  __shared__ double red[kChainThreads / 64];
+  __shared__ int acc_flag;
+  const int c = blockIdx.x;
+  double plp = 0.0;
+  int bad = 0;
+  for (int d = threadIdx.x; d < a.D; d += kChainThreads) {
+    const float zv = a.z[(int64_t)c * a.D_pad + d];
+    plp += prior_logp(a, d, zv);
+    if (!isfinite(zv)) bad = 1;
+  }
+  plp = block_sum_f64<kChainThreads>(plp, red);
+  const double nbad = block_sum_f64<kChainThreads>((double)bad, red);
+  if (threadIdx.x == 0) {
+    const double lpn = (double)a.lik_weight * a.ll[c] + plp;
+    const double ratio = lpn - a.logp[c];
+    const double u = a.u_fed ? a.u_fed[c] : accept_uniform(a.seed, a.counter, (uint32_t)c);
+    const int acc = (log(u) < ratio) && nbad == 0.0;
+    acc_flag = acc;
+    if (acc) a.logp[c] = lpn;
+    if (a.accepted_out) a.accepted_out[c] = acc;
+    if (a.n_accept) a.n_accept[c] += acc;
+    if (a.ndiv && (nbad != 0.0 || !(ratio == ratio))) a.ndiv[c] += 1;
+  }
+  __syncthreads();
+  const int acc = acc_flag;
+  for (int d = threadIdx.x; d < a.D; d += kChainThreads) {
+    const int64_t idx = (int64_t)c * a.D_pad + d;
+    float zv = a.z[idx];
+    if (!acc) {
+      zv = a.z0[idx];
+      a.z[idx] = zv;
+    }
+    if (a.sample_out) a.sample_out[(int64_t)c * a.D + d] = zv;
+  }
+}
+
 __global__ void copy_out_kernel(const float* __restrict__ src, int C, int D, int D_pad,
                                 float* __restrict__ dst) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1504,6 +1567,40 @@ static int sg_step(edhmc_ctx* ctx, bool hamiltonian, float lr, float friction, f
   EDHMC_TRY(check_launch(ctx, "sg_step"));
   ctx->g_valid = false;   // positions moved; the next step re-evaluates the gradient
   return EDHMC_OK;
+}
+
+int edhmc_mh_step(edhmc_ctx* ctx, const float* proposal_scale, const double* u, uint64_t seed,
+                  uint64_t counter, int32_t* accepted_out, float* sample_out, int32_t* n_accept,
+                  void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  if (!ctx->bound || !ctx->state_valid)
+    return set_err(ctx, EDHMC_ESTATE, "no chain state: call edhmc_set_state first");
+  if (!proposal_scale) return set_err(ctx, EDHMC_EINVAL, "proposal_scale is NULL");
+  // the carried log joint must come from the forward formula the ratio uses
+  if (ctx->g_valid) EDHMC_TRY(glm_refresh_logp(ctx, st));
+  surrogate_invalidate(ctx->sur);
+  ChainArgs a = chain_args(ctx);
+  a.seed = seed;
+  a.counter = counter;
+  a.u_fed = u;
+  {
+    ProfScope ps(ctx, PK_CHAIN, st);
+    hipLaunchKernelGGL(mh_propose_kernel, dim3(ctx->C), dim3(kChainThreads), 0, st, a,
+                       proposal_scale);
+  }
+  EDHMC_TRY(check_launch(ctx, "mh_propose"));
+  ctx->g_valid = false;
+  EDHMC_TRY(glm_forward_ll(ctx, st));
+  a.sample_out = sample_out;
+  a.accepted_out = accepted_out;
+  a.n_accept = n_accept;
+  {
+    ProfScope ps(ctx, PK_CHAIN, st);
+    hipLaunchKernelGGL(mh_accept_kernel, dim3(ctx->C), dim3(kChainThreads), 0, st, a);
+  }
+  return check_launch(ctx, "mh_accept");
 }
 
 int edhmc_set_lik_weight(edhmc_ctx* ctx, float w) {

@@ -15,7 +15,10 @@ from edward_amd import _lib as L
 def _dev_tensor(x, dtype, device):
   if isinstance(x, torch.Tensor):
     return x.to(device=device, dtype=dtype).contiguous()
-  return torch.as_tensor(np.asarray(x), dtype=dtype, device=device).contiguous()
+  a = np.asarray(x)
+  if not a.flags.writeable:
+    a = a.copy()      # broadcast views are read-only; torch wants its own buffer
+  return torch.as_tensor(a, dtype=dtype, device=device).contiguous()
 
 
 def jittered_step(step_size, jitter, seed, counter):
@@ -132,6 +135,18 @@ class GLMEngine(Context):
   def sgld_step(self, learning_rate, seed=0, counter=0, sample_out=None, n_accept=None):
     self.check(self.lib.edhmc_sgld_step(self.h, float(learning_rate), int(seed), int(counter),
                                         L.ptr(sample_out), L.ptr(n_accept), self.stream))
+
+  def mh_step(self, proposal_scale, uniforms=None, seed=0, counter=0, sample_out=None,
+              n_accept=None, accepted_out=None):
+    """Random-walk Metropolis-Hastings step with proposal N(z, diag(s^2))."""
+    s = _dev_tensor(np.broadcast_to(np.asarray(
+        proposal_scale.cpu() if isinstance(proposal_scale, torch.Tensor) else proposal_scale,
+        np.float32), (self.D,)), torch.float32, self.device)
+    self._mh_scale = s
+    u = None if uniforms is None else _dev_tensor(uniforms, torch.float64, self.device)
+    self.check(self.lib.edhmc_mh_step(self.h, L.ptr(s), L.ptr(u), int(seed), int(counter),
+                                      L.ptr(accepted_out), L.ptr(sample_out), L.ptr(n_accept),
+                                      self.stream))
 
   def sghmc_step(self, learning_rate, friction, v, seed=0, counter=0, sample_out=None,
                  n_accept=None):

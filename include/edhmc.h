@@ -151,6 +151,18 @@ int edhmc_sgld_step(edhmc_ctx* ctx, float learning_rate, uint64_t seed, uint64_t
 int edhmc_sghmc_step(edhmc_ctx* ctx, float learning_rate, float friction, float* v,
                      uint64_t seed, uint64_t counter, float* sample_out, int32_t* n_accept,
                      void* stream);
+/* Random-walk Metropolis-Hastings over the bound GLM
+ * (edward/inferences/metropolis_hastings.py:64-156) with the proposal
+ * Normal(loc = z, scale = proposal_scale[D]) (device fp32): z' = z + s xi
+ * (xi from the momentum stream of (seed, counter)), ratio = log p(x, z') -
+ * log p(x, z) (the symmetric proposal's terms cancel), accept iff
+ * log u < ratio with u [C] fp64 fed or drawn on the device.  The log joint is
+ * the forward pass's (bf16: hi/lo split, FP64 reduction).  Outputs as
+ * edhmc_transition. */
+int edhmc_mh_step(edhmc_ctx* ctx, const float* proposal_scale, const double* u, uint64_t seed,
+                  uint64_t counter, int32_t* accepted_out, float* sample_out, int32_t* n_accept,
+                  void* stream);
+
 /* Multiplier of the likelihood term in the log joint and its gradient:
  * Edward's `scale` of the observed variable (inference.py:187-193, used by
  * sgld.py:95-121 and sghmc.py:99-135 for minibatches; HMC ignores it,

@@ -443,6 +443,19 @@ def sghmc_step(z, v, grad, lr, friction, xi):
   return z + v, (1.0 - 0.5 * friction) * v + lr * grad + math.sqrt(lr * friction) * xi
 
 
+def mh_step(z, logp_z, scale, xi, u, logp_fn):
+  """Random-walk Metropolis-Hastings (``metropolis_hastings.py:97-156``) with
+  proposal N(z, diag(scale^2)): z' = z + scale xi; the symmetric proposal
+  terms cancel, ratio = logp(z') - logp(z); accept iff log u < ratio.
+  Returns (z_new, logp_new, accept, ratio)."""
+  zp = z + scale * xi
+  lp = logp_fn(zp)
+  ratio = lp - logp_z
+  with np.errstate(divide='ignore'):
+    acc = accept_rule(np.log(u), ratio)
+  return (np.where(acc[:, None], zp, z), np.where(acc, lp, logp_z), acc, ratio)
+
+
 def scaled_logp_grad(logp_grad_lik, prior_grad, lik_weight, prior_weight):
   """Gradient of ``inference.py:187-193``'s scaled log joint: lik_weight times
   the likelihood gradient plus prior_weight times the prior gradient."""

@@ -395,3 +395,20 @@ def test_sghmc_gaussian_stationary_moments():
   P = linalg.solve_discrete_lyapunov(A, Q)
   assert abs(z.var() / P[0, 0] - 1.0) < 0.03
   assert abs(v.var() / P[1, 1] - 1.0) < 0.03
+
+
+def test_mh_random_walk_targets_the_gaussian():
+  """Random-walk MH on N(0, s^2 I) keeps the target: the stationary variance
+  of many chains after burn-in is s^2; the symmetric proposal's density terms
+  cancel, so the oracle's ratio is the log-density difference alone."""
+  rng = np.random.default_rng(7)
+  s2, C, D = 0.5, 20000, 2
+  logp = lambda z: -0.5 * (z ** 2).sum(axis=1) / s2
+  z = np.zeros((C, D))
+  lz = logp(z)
+  acc_tot = 0
+  for _ in range(300):
+    z, lz, acc, _ = O.mh_step(z, lz, 0.6, rng.normal(size=(C, D)), rng.random(C), logp)
+    acc_tot += acc.mean()
+  assert abs(z.var() / s2 - 1.0) < 0.03
+  assert 0.2 < acc_tot / 300 < 0.95
