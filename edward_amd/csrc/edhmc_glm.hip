@@ -131,6 +131,30 @@ struct EpiGLM {
       gj[j] = (goff && valid[j]) ? gidx[n] : 0;
     }
     const int ml0 = p.m0 - p.tm * BM, nl0 = p.n0 - p.tn * BN;   // tile-local origin
+    // Group intercepts: rows are sorted by group, so a row tile spans the
+    // groups g_lo..g_hi; when that range is short, the tile's intercepts
+    // a[m][g_lo..g_hi] are staged in LDS once (one coalesced pass) instead of
+    // one scattered global load per accumulator element.
+    constexpr int KG = 8;
+    float* gl = reinterpret_cast<float*>(p.smem + (stage ? RTILE : 0) + LLB);
+    int g_lo = 0;
+    bool g_lds = false;
+    if (goff) {
+      const int64_t nf = (int64_t)p.tn * BN;
+      const int64_t nl = min((int64_t)N, nf + BN) - 1;
+      if (nf < N) {
+        g_lo = gidx[nf];
+        const int ng = gidx[nl] - g_lo + 1;
+        g_lds = ng <= KG && (stage ? RTILE : 0) + LLB + BM * KG * 4 <= p.smem_bytes;
+        if (g_lds) {
+          for (int t = threadIdx.x; t < BM * ng; t += NT) {
+            const int m = t / ng, k = t - m * ng;
+            gl[m * KG + k] = goff[(int64_t)(p.tm * BM + m) * goff_ld + g_lo + k];
+          }
+          __syncthreads();
+        }
+      }
+    }
     // Forward-only Bernoulli pass: the linear part sum_n y_n eta_n = w . X^T y
     // is added per chain in FP64 after the reduction (ll_linear_kernel), so the
     // epilogue sums only -softplus(eta) (fp32 over the lane's WN columns, FP64
@@ -152,7 +176,13 @@ struct EpiGLM {
 #pragma unroll
           for (int j = 0; j < WN; ++j) {
             const int nc = 32 * j + (p.lane & 31);
-            const float eta = acc[i][j][r] + (grow ? grow[gj[j]] : 0.0f);
+            // padded rows (n >= N) carry gj = 0: index slot 0 for them, never
+            // gj - g_lo, which is negative there and would address outside
+            // the workgroup's LDS
+            const float eta = acc[i][j][r] +
+                              (grow ? (g_lds ? gl[(ml0 + mr) * KG + (valid[j] ? gj[j] - g_lo : 0)]
+                                             : grow[gj[j]])
+                                    : 0.0f);
             float dll;
             if (LL) {
               double ll;

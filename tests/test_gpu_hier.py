@@ -213,3 +213,19 @@ def test_hier_edward_surface():
   lp, _ = inf.engine.logp_grad(vals)
   lp_ref, _ = m.logp_grad(vals.astype(np.float64))
   np.testing.assert_allclose(lp.cpu().numpy(), lp_ref, rtol=1e-6, atol=1e-3)
+
+
+def test_hier_many_groups_partial_last_tile():
+  """1600 groups of 16-24 rows: every row tile spans few groups (the
+  epilogue stages their intercepts in LDS) and the partial last tile starts
+  at a group index near 1600, where the padded rows must not index the
+  staged tile with gidx - g_lo < 0."""
+  sizes = list(np.random.default_rng(41).integers(16, 25, size=1600))
+  X, y, off = _data(42, sizes, 8)
+  eng = _engine(X, y, off)
+  m = O.HierLogReg(X.astype(np.float64), y, off)
+  th = (np.random.default_rng(43).normal(size=(5, m.dim)) * 0.3).astype(np.float32)
+  lp, g = eng.logp_grad(th)
+  lp_ref, g_ref = m.logp_grad(th.astype(np.float64))
+  assert np.all(np.abs(lp.cpu().numpy() - lp_ref) <= 1e-6 * np.abs(lp_ref) + 1e-3)
+  assert np.abs(g.cpu().numpy() - g_ref).max() <= 1e-4 * np.abs(g_ref).max()
