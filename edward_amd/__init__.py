@@ -14,7 +14,7 @@ from edward_amd import models  # noqa: F401
 from edward_amd import util  # noqa: F401
 from edward_amd.criticisms import copy, evaluate, ppc  # noqa: F401
 from edward_amd.inferences import (HMC, SGHMC, SGLD, Inference, MetropolisHastings,  # noqa: F401
-                                   MonteCarlo)
+                                   MonteCarlo, ReplicaExchangeMC)
 from edward_amd.models import dot, exp, gather, placeholder  # noqa: F401
 from edward_amd.util import get_seed, get_session, set_seed  # noqa: F401
 from edward_amd.version import __version__  # noqa: F401

@@ -50,6 +50,8 @@ SIGNATURES = {
     'edhmc_sghmc_step': (ctypes.c_int, [_c_p, _f32, _f32, _c_p, _u64, _u64, _c_p, _c_p, _c_p]),
     'edhmc_set_lik_weight': (ctypes.c_int, [_c_p, _f32]),
     'edhmc_mh_step': (ctypes.c_int, [_c_p, _c_p, _c_p, _u64, _u64, _c_p, _c_p, _c_p, _c_p]),
+    'edhmc_replica_exchange_step': (ctypes.c_int, [_c_p, _c_p, _c_p, _i32, _f32, _u64, _u64,
+                                                   _c_p, _c_p, _c_p, _c_p, _c_p]),
     'edhmc_transition': (ctypes.c_int, [_c_p, _f32, _i32, _c_p, _c_p, _u64, _u64, _c_p,
                                         _c_p, _c_p, _c_p]),
     'edhmc_run': (ctypes.c_int, [_c_p, _f32, _i32, _c_p, _i64, _i64, _i64, _u64, _u64,

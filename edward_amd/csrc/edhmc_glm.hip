@@ -499,6 +499,7 @@ struct ChainArgs {
   const float* Gpart; int ksplit;
   const double* ll;         // [C_pad] reduced log-likelihood
   float lik_weight;         // multiplies the likelihood (scale of the SG samplers; 1 for HMC)
+  const float* beta;        // per-chain inverse temperature of the MH ratio (null = 1)
   float* sample_out; int32_t* accepted_out; int32_t* n_accept; int32_t* ndiv;
   double* mom_mean; double* mom_m2; int64_t mom_n;   // mom_n = count after update
   float* grad_out; double* logp_out;
@@ -820,7 +821,9 @@ __global__ void __launch_bounds__(kChainThreads) mh_accept_kernel(ChainArgs a) {
   const double nbad = block_sum_f64<kChainThreads>((double)bad, red);
   if (threadIdx.x == 0) {
     const double lpn = (double)a.lik_weight * a.ll[c] + plp;
-    const double ratio = lpn - a.logp[c];
+    // replica_exchange_mc.py:200: the whole MH ratio is scaled by the
+    // replica's inverse temperature
+    const double ratio = (lpn - a.logp[c]) * (a.beta ? (double)a.beta[c] : 1.0);
     const double u = a.u_fed ? a.u_fed[c] : accept_uniform(a.seed, a.counter, (uint32_t)c);
     const int acc = (log(u) < ratio) && nbad == 0.0;
     acc_flag = acc;
@@ -1599,12 +1602,104 @@ static int sg_step(edhmc_ctx* ctx, bool hamiltonian, float lr, float friction, f
   return EDHMC_OK;
 }
 
+static int mh_step(edhmc_ctx* ctx, const float* proposal_scale, const float* beta,
+                   const double* u, uint64_t seed, uint64_t counter, int32_t* accepted_out,
+                   float* sample_out, int32_t* n_accept, hipStream_t st);
+
 int edhmc_mh_step(edhmc_ctx* ctx, const float* proposal_scale, const double* u, uint64_t seed,
                   uint64_t counter, int32_t* accepted_out, float* sample_out, int32_t* n_accept,
                   void* stream) {
   if (!ctx) return EDHMC_EINVAL;
   DeviceGuard dg(ctx->device);
+  return mh_step(ctx, proposal_scale, nullptr, u, seed, counter, accepted_out, sample_out,
+                 n_accept, (hipStream_t)stream);
+}
+
+// Adjacent-replica exchange (replica_exchange_mc.py:214-237), for each of the
+// C / R independent replica ladders (chain = replica * n_ladders + ladder):
+// with probability exchange_freq, from a random start i0 in {0, 1}, pairs
+// (i, i+1), (i+2, i+3), ... swap their states iff
+// log u < (logp_i - logp_{i+1}) (beta_{i+1} - beta_i), all decided on the
+// log joints before any swap.  Draws: Philox on the uniform stream of
+// (seed, counter) with a = 1 + ladder: chain slot 0 gives the coin
+// ((x0 >> 8) 2^-24) and the start (x1 & 1), slot 1 + i the pair's uniform
+// (53 bits, as the accept uniforms).
+__global__ void __launch_bounds__(kChainThreads)
+replica_exchange_kernel(ChainArgs a, int R, int ladders, const float* __restrict__ beta,
+                        float exchange_freq, int32_t* __restrict__ swaps) {
+  const int l = blockIdx.x;
+  __shared__ int do_swap[64];
+  if (threadIdx.x < R) do_swap[threadIdx.x] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const u32x4 x = philox_draw(a.seed, a.counter, 1u + (uint32_t)l, 0u, 1);
+    const float coin = (float)(x.x >> 8) * 5.9604644775390625e-8f;
+    if (coin < exchange_freq) {
+      for (int i = (int)(x.y & 1u); i + 1 < R; i += 2) {
+        const int ci = i * ladders + l, cj = (i + 1) * ladders + l;
+        const double ratio = (a.logp[ci] - a.logp[cj]) * ((double)beta[cj] - (double)beta[ci]);
+        const u32x4 y = philox_draw(a.seed, a.counter, 1u + (uint32_t)l, 1u + (uint32_t)i, 1);
+        const uint64_t mbits = ((uint64_t)y.x << 21) | (uint64_t)(y.y >> 11);
+        const double u = (double)mbits * 1.1102230246251565e-16;   // 2^-53
+        if (log(u) < ratio) do_swap[i] = 1;
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = 0; i + 1 < R; ++i) {
+    if (!do_swap[i]) continue;
+    const int ci = i * ladders + l, cj = (i + 1) * ladders + l;
+    for (int d = threadIdx.x; d < a.D; d += kChainThreads) {
+      const int64_t pi = (int64_t)ci * a.D_pad + d, pj = (int64_t)cj * a.D_pad + d;
+      const float t = a.z[pi];
+      a.z[pi] = a.z[pj];
+      a.z[pj] = t;
+    }
+    if (threadIdx.x == 0) {
+      const double t = a.logp[ci];
+      a.logp[ci] = a.logp[cj];
+      a.logp[cj] = t;
+      if (swaps) swaps[l] += 1;
+    }
+  }
+  __syncthreads();
+  if (a.sample_out)
+    for (int i = 0; i < R; ++i) {
+      const int c = i * ladders + l;
+      for (int d = threadIdx.x; d < a.D; d += kChainThreads)
+        a.sample_out[(int64_t)c * a.D + d] = a.z[(int64_t)c * a.D_pad + d];
+    }
+}
+
+int edhmc_replica_exchange_step(edhmc_ctx* ctx, const float* proposal_scale, const float* beta,
+                                int32_t n_replica, float exchange_freq, uint64_t seed,
+                                uint64_t counter, int32_t* accepted_out, float* sample_out,
+                                int32_t* n_accept, int32_t* swaps, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
   hipStream_t st = (hipStream_t)stream;
+  if (!beta) return set_err(ctx, EDHMC_EINVAL, "beta is NULL");
+  if (n_replica < 1 || n_replica > 64 || !ctx->C || ctx->C % n_replica)
+    return set_err(ctx, EDHMC_EINVAL, "chains must be n_replica (1..64) ladders of equal size");
+  EDHMC_TRY(mh_step(ctx, proposal_scale, beta, nullptr, seed, counter, accepted_out, nullptr,
+                    n_accept, st));
+  ChainArgs a = chain_args(ctx);
+  a.seed = seed;
+  a.counter = counter;
+  a.sample_out = sample_out;
+  const int ladders = ctx->C / n_replica;
+  {
+    ProfScope ps(ctx, PK_CHAIN, st);
+    hipLaunchKernelGGL(replica_exchange_kernel, dim3(ladders), dim3(kChainThreads), 0, st, a,
+                       n_replica, ladders, beta, exchange_freq, swaps);
+  }
+  EDHMC_TRY(check_launch(ctx, "replica_exchange"));
+  return EDHMC_OK;
+}
+
+static int mh_step(edhmc_ctx* ctx, const float* proposal_scale, const float* beta,
+                   const double* u, uint64_t seed, uint64_t counter, int32_t* accepted_out,
+                   float* sample_out, int32_t* n_accept, hipStream_t st) {
   if (!ctx->bound || !ctx->state_valid)
     return set_err(ctx, EDHMC_ESTATE, "no chain state: call edhmc_set_state first");
   if (!proposal_scale) return set_err(ctx, EDHMC_EINVAL, "proposal_scale is NULL");
@@ -1615,6 +1710,7 @@ int edhmc_mh_step(edhmc_ctx* ctx, const float* proposal_scale, const double* u, 
   a.seed = seed;
   a.counter = counter;
   a.u_fed = u;
+  a.beta = beta;
   {
     ProfScope ps(ctx, PK_CHAIN, st);
     hipLaunchKernelGGL(mh_propose_kernel, dim3(ctx->C), dim3(kChainThreads), 0, st, a,

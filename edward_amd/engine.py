@@ -148,6 +148,21 @@ class GLMEngine(Context):
                                       L.ptr(accepted_out), L.ptr(sample_out), L.ptr(n_accept),
                                       self.stream))
 
+  def replica_exchange_step(self, proposal_scale, beta, n_replica, exchange_freq, seed=0,
+                            counter=0, sample_out=None, n_accept=None, accepted_out=None,
+                            swaps=None):
+    """Tempered random-walk MH on every chain, then adjacent-replica exchanges
+    (chains ordered replica-major: chain = replica * ladders + ladder)."""
+    s = _dev_tensor(np.broadcast_to(np.asarray(
+        proposal_scale.cpu() if isinstance(proposal_scale, torch.Tensor) else proposal_scale,
+        np.float32), (self.D,)), torch.float32, self.device)
+    b = _dev_tensor(beta, torch.float32, self.device)
+    self._re_keep = (s, b)
+    self.check(self.lib.edhmc_replica_exchange_step(
+        self.h, L.ptr(s), L.ptr(b), int(n_replica), float(exchange_freq), int(seed),
+        int(counter), L.ptr(accepted_out), L.ptr(sample_out), L.ptr(n_accept), L.ptr(swaps),
+        self.stream))
+
   def sghmc_step(self, learning_rate, friction, v, seed=0, counter=0, sample_out=None,
                  n_accept=None):
     self.check(self.lib.edhmc_sghmc_step(self.h, float(learning_rate), float(friction), L.ptr(v),

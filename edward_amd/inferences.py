@@ -467,3 +467,80 @@ class MetropolisHastings(MonteCarlo):
                         n_accept=self._n_accept)
     self.program.scatter(t, self._row)
     self._state_t = t
+
+
+class ReplicaExchangeMC(MetropolisHastings):
+  """Replica-exchange Monte Carlo (``edward/inferences/replica_exchange_mc.py:
+  14-276``): replica i runs random-walk Metropolis-Hastings on the log joint
+  tempered by ``inverse_temperatures[i]`` (the first must be 1); with
+  probability ``exchange_freq`` per update adjacent replicas exchange states
+  (Metropolis criterion on the untempered log joints).  Every replica keeps
+  its own ``Empirical`` store (``replica_vars``, zeros at the start, where
+  each replica's chain starts); the user's store receives replica 0.  A
+  ``[T, C, D]`` store runs C independent replica ladders."""
+
+  def __init__(self, latent_vars, proposal_vars, data=None,
+               inverse_temperatures=np.logspace(0, -2, 5), exchange_freq=0.1):
+    betas = np.asarray(inverse_temperatures, np.float64).reshape(-1)
+    if betas.size < 1 or betas[0] != 1:
+      raise ValueError('inverse_temperatures[0] must be 1.')
+    super(ReplicaExchangeMC, self).__init__(latent_vars, proposal_vars, data)
+    self.n_replica = int(betas.size)
+    self.inverse_temperatures = betas
+    self.exchange_freq = float(exchange_freq)
+    self.replica_vars = [{z: M.Empirical(torch.zeros_like(qz.params)) for z, qz in
+                          self.latent_vars.items()} for _ in range(self.n_replica)]
+
+  def initialize(self, *args, **kwargs):
+    """``replica_exchange_mc.py:74-76`` (auto_transform off)."""
+    kwargs['auto_transform'] = False
+    return super(ReplicaExchangeMC, self).initialize(*args, **kwargs)
+
+  def _on_compiled(self):
+    super(ReplicaExchangeMC, self)._on_compiled()
+    R, C = self.n_replica, self.program.n_chains
+    self._beta = torch.as_tensor(np.repeat(self.inverse_temperatures, C).astype(np.float32),
+                                 device=self._device)
+    self._row_all = torch.empty(R * C, self.program.dim, device=self._device)
+    self._n_accept_all = torch.zeros(R * C, dtype=torch.int32, device=self._device)
+    self._swaps = torch.zeros(C, dtype=torch.int32, device=self._device)
+
+  def _reset_counts(self):
+    super(ReplicaExchangeMC, self)._reset_counts()
+    if getattr(self, '_n_accept_all', None) is not None:
+      self._n_accept_all.zero_()
+
+  def _gather_replicas(self, t):
+    C = self.program.n_chains
+    rows = []
+    for rep in self.replica_vars:
+      rows.append(torch.cat([rep[l.rv].params[t].reshape(C, l.k).to(torch.float32)
+                             for l in self.program.latents], dim=1))
+    return torch.cat(rows, dim=0).contiguous()
+
+  def _scatter_replicas(self, t, rows):
+    C = self.program.n_chains
+    for i, rep in enumerate(self.replica_vars):
+      block = rows[i * C:(i + 1) * C]
+      for l in self.program.latents:
+        dst = rep[l.rv].params[t]
+        dst.copy_(block[:, l.offset:l.offset + l.k].reshape(dst.shape))
+    self.program.scatter(t, rows[:C])
+
+  def _step(self):
+    t = self.t
+    need = max(t - 1, 0)
+    if self._state_t != need:
+      self.engine.set_state(self._gather_replicas(need))
+    self.engine.replica_exchange_step(self._proposal_scale, self._beta, self.n_replica,
+                                      self.exchange_freq, seed=self.seed, counter=t,
+                                      sample_out=self._row_all, n_accept=self._n_accept_all,
+                                      swaps=self._swaps)
+    self._n_accept.copy_(self._n_accept_all[:self.program.n_chains])
+    self._scatter_replicas(t, self._row_all)
+    self._state_t = t
+
+  @property
+  def n_swaps(self):
+    """Exchanges per replica ladder since the start (no reference counterpart)."""
+    return self._swaps.cpu().numpy()

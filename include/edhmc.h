@@ -163,6 +163,23 @@ int edhmc_mh_step(edhmc_ctx* ctx, const float* proposal_scale, const double* u, 
                   uint64_t counter, int32_t* accepted_out, float* sample_out, int32_t* n_accept,
                   void* stream);
 
+/* Replica-exchange Monte Carlo (edward/inferences/replica_exchange_mc.py:
+ * 78-237).  The C chains form n_replica replicas of C / n_replica ladders
+ * (chain = replica * ladders + ladder); beta [C] (device fp32) is each
+ * chain's inverse temperature.  Each chain takes the random-walk MH step of
+ * edhmc_mh_step with its ratio scaled by beta (replica_exchange_mc.py:200);
+ * then, per ladder, with probability exchange_freq and from a random start
+ * in {0, 1}, adjacent replicas (i, i+1), (i+2, i+3), ... exchange states iff
+ * log u < (logp_i - logp_{i+1}) (beta_{i+1} - beta_i) (:214-237).  Draws: the
+ * MH proposal and accept as edhmc_mh_step; the exchange coin, start and
+ * uniforms from the uniform stream with a = 1 + ladder.  sample_out [C, D]
+ * gets every chain's state after the exchange, n_accept [C] the MH accepts,
+ * swaps [C / n_replica] int32 the exchanges (any output may be NULL). */
+int edhmc_replica_exchange_step(edhmc_ctx* ctx, const float* proposal_scale, const float* beta,
+                                int32_t n_replica, float exchange_freq, uint64_t seed,
+                                uint64_t counter, int32_t* accepted_out, float* sample_out,
+                                int32_t* n_accept, int32_t* swaps, void* stream);
+
 /* Multiplier of the likelihood term in the log joint and its gradient:
  * Edward's `scale` of the observed variable (inference.py:187-193, used by
  * sgld.py:95-121 and sghmc.py:99-135 for minibatches; HMC ignores it,

@@ -443,17 +443,47 @@ def sghmc_step(z, v, grad, lr, friction, xi):
   return z + v, (1.0 - 0.5 * friction) * v + lr * grad + math.sqrt(lr * friction) * xi
 
 
-def mh_step(z, logp_z, scale, xi, u, logp_fn):
+def mh_step(z, logp_z, scale, xi, u, logp_fn, beta=1.0):
   """Random-walk Metropolis-Hastings (``metropolis_hastings.py:97-156``) with
   proposal N(z, diag(scale^2)): z' = z + scale xi; the symmetric proposal
-  terms cancel, ratio = logp(z') - logp(z); accept iff log u < ratio.
-  Returns (z_new, logp_new, accept, ratio)."""
+  terms cancel, ratio = beta (logp(z') - logp(z)) (beta = the replica's
+  inverse temperature, ``replica_exchange_mc.py:200``; 1 for plain MH);
+  accept iff log u < ratio.  Returns (z_new, logp_new, accept, ratio)."""
   zp = z + scale * xi
   lp = logp_fn(zp)
-  ratio = lp - logp_z
+  ratio = (lp - logp_z) * beta
   with np.errstate(divide='ignore'):
     acc = accept_rule(np.log(u), ratio)
   return (np.where(acc[:, None], zp, z), np.where(acc, lp, logp_z), acc, ratio)
+
+
+def replica_exchange(logp, betas, ladders, seed, counter, exchange_freq):
+  """The exchange move of ``replica_exchange_mc.py:214-237`` for chains
+  ordered replica-major (chain = replica * ladders + ladder): per ladder, a
+  coin u0 = (x0 >> 8) 2^-24 < exchange_freq and a start i0 = x1 & 1 from
+  Philox(ctr=(1 + ladder, 0, counter, stream 1)); pair (i, i+1) swaps iff
+  log u < (logp_i - logp_{i+1}) (beta_{i+1} - beta_i) with u the 53-bit
+  uniform of Philox(ctr=(1 + ladder, 1 + i, counter, stream 1)).  Returns the
+  permutation new_index[chain] (the chain whose state each chain takes)."""
+  R = len(betas)
+  perm = np.arange(R * ladders)
+  ll = np.arange(ladders, dtype=np.uint64)
+  keys = np.broadcast_to(_key(seed), (ladders, 2))
+  x = philox4x32_10(_ctr(1 + ll, np.zeros(ladders, np.uint64), counter, STREAM_UNIFORM), keys)
+  active = np.floor(x[:, 0].astype(np.float64) / 256.0) * 2.0 ** -24 < exchange_freq
+  start = (x[:, 1] & 1).astype(np.int64)
+  for i in range(R - 1):
+    y = philox4x32_10(_ctr(1 + ll, np.full(ladders, 1 + i, np.uint64), counter,
+                           STREAM_UNIFORM), keys)
+    u = ((y[:, 0].astype(np.uint64) << np.uint64(21)) |
+         (y[:, 1].astype(np.uint64) >> np.uint64(11))).astype(np.float64) * 2.0 ** -53
+    ci = i * ladders + np.arange(ladders)
+    cj = ci + ladders
+    ratio = (logp[ci] - logp[cj]) * (betas[i + 1] - betas[i])
+    with np.errstate(divide='ignore'):
+      swap = active & ((i - start) % 2 == 0) & (i >= start) & (np.log(u) < ratio)
+    perm[ci[swap]], perm[cj[swap]] = cj[swap], ci[swap]
+  return perm
 
 
 def scaled_logp_grad(logp_grad_lik, prior_grad, lik_weight, prior_weight):

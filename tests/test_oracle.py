@@ -412,3 +412,28 @@ def test_mh_random_walk_targets_the_gaussian():
     acc_tot += acc.mean()
   assert abs(z.var() / s2 - 1.0) < 0.03
   assert 0.2 < acc_tot / 300 < 0.95
+
+
+def test_replica_exchange_keeps_the_tempered_targets():
+  """MH at inverse temperatures (1, 0.25) on N(0, 1) plus the exchange move:
+  replica i must stay distributed as N(0, 1/beta_i) (the exchange ratio's
+  sign or temperature factor wrong would distort both), and exchanges must
+  happen."""
+  rng = np.random.default_rng(11)
+  betas = np.array([1.0, 0.25])
+  L = 4000
+  C = 2 * L
+  beta_c = np.repeat(betas, L)
+  logp = lambda z: -0.5 * (z ** 2).sum(axis=1)
+  z = np.zeros((C, 1))
+  lz = logp(z)
+  swaps = 0
+  for t in range(400):
+    z, lz, _, _ = O.mh_step(z, lz, 1.2 / np.sqrt(beta_c)[:, None], rng.normal(size=(C, 1)),
+                            rng.random(C), logp, beta_c)
+    perm = O.replica_exchange(lz, betas, L, 5, t, 0.5)
+    swaps += int((perm != np.arange(C)).sum())
+    z, lz = z[perm], lz[perm]
+  assert abs(z[:L].var() - 1.0) < 0.08
+  assert abs(z[L:].var() - 4.0) < 0.35
+  assert swaps > 0
