@@ -73,6 +73,8 @@ struct edhmc_ctx {
   int64_t N = 0, N_pad = 0;
   int D = 0, D_pad = 0;
   edhmc::DevBuf Xr, Xc, y, ploc, pscale;
+  edhmc::DevBuf pkind, plnorm;   // constrained prior families (has_kinds)
+  bool has_kinds = false;
 
   // ---- chains ----
   int C = 0, C_pad = 0;

@@ -18,6 +18,7 @@
 #include "ctx.h"
 #include "gemm_nt.h"
 #include "philox.h"
+#include "transform.h"
 
 namespace edhmc {
 
@@ -493,6 +494,8 @@ struct ChainArgs {
   void* z_lp;
   double* logp; double* K0;
   const float* ploc; const float* pscale;
+  const uint8_t* pkind;     // per-coordinate prior family | operand << 4 (null: Normal, identity)
+  const double* plnorm;     // per-coordinate normalising constant of that family
   float half, full;         // fl32(eps/2), fl32(eps)
   const float* r_fed; const double* u_fed;
   uint64_t seed, counter;
@@ -507,7 +510,15 @@ struct ChainArgs {
   float* rec_W; float* rec_G;   // training-pair rows for this step (or null)
 };
 
+// The linear predictor's operand for position zv of coordinate d.
+__device__ inline float operand(const ChainArgs& a, int d, float zv) {
+  if (!a.pkind) return zv;
+  const int op = a.pkind[d] >> 4;
+  return op == kOpIdentity ? zv : (float)operand_f64(op, (double)zv);
+}
+
 __device__ inline void write_lp(const ChainArgs& a, int c, int d, float zv) {
+  zv = operand(a, d, zv);
   const int64_t idx = (int64_t)c * a.D_pad + d;
   if (a.bf16) {
     bf16_t* p = (bf16_t*)a.z_lp;
@@ -526,9 +537,22 @@ __device__ inline float prior_grad(const ChainArgs& a, int d, float zv) {
 }
 
 __device__ inline double prior_logp(const ChainArgs& a, int d, float zv) {
+  if (a.pkind)
+    return prior_term(a.pkind[d] & 15, (double)zv, a.ploc[d], a.pscale[d], a.plnorm[d]).lp;
   const double s = a.pscale[d];
   const double t = ((double)zv - (double)a.ploc[d]) / s;
   return -0.5 * t * t - log(s) - 0.5 * kLog2Pi;
+}
+
+// Gradient of the log joint in position space: gl = d loglik / d operand
+// (already weighted), chained through the operand map, plus the prior term
+// (with the log-Jacobian of a constrained prior).
+__device__ inline float grad_u(const ChainArgs& a, int d, float zv, float gl) {
+  if (!a.pkind) return gl + prior_grad(a, d, zv);
+  const int k = a.pkind[d];
+  const double u = zv;
+  const PriorTerm t = prior_term(k & 15, u, a.ploc[d], a.pscale[d], a.plnorm[d]);
+  return (float)((double)gl * d_operand_f64(k >> 4, u) + t.dlp);
 }
 
 // Momentum draw, K0 = |r0|^2 / 2, save (z0, g0), first half kick and drift.
@@ -585,7 +609,7 @@ __global__ void __launch_bounds__(kChainThreads) chain_mid_kernel(ChainArgs a) {
   for (int d = threadIdx.x; d < a.D; d += kChainThreads) {
     const int64_t idx = (int64_t)c * a.D_pad + d;
     const float zv = a.z[idx];
-    const float gv = gsum(a, c, d) + prior_grad(a, d, zv);
+    const float gv = grad_u(a, d, zv, gsum(a, c, d));
     a.g[idx] = gv;
     if (a.rec_W) {
       a.rec_W[(int64_t)c * a.D + d] = zv;
@@ -662,7 +686,7 @@ __device__ inline void chain_accept(const ChainArgs& a, GradAt g_at) {
 
 __global__ void __launch_bounds__(kChainThreads) chain_last_kernel(ChainArgs a) {
   chain_accept(a, [&](int c, int d, int64_t, float zv) {
-    return gsum(a, c, d) + prior_grad(a, d, zv);
+    return grad_u(a, d, zv, gsum(a, c, d));
   });
 }
 
@@ -675,7 +699,7 @@ __global__ void __launch_bounds__(kChainThreads) chain_eval_kernel(ChainArgs a) 
   for (int d = threadIdx.x; d < a.D; d += kChainThreads) {
     const int64_t idx = (int64_t)c * a.D_pad + d;
     const float zv = a.z[idx];
-    const float gv = gsum(a, c, d) + prior_grad(a, d, zv);
+    const float gv = grad_u(a, d, zv, gsum(a, c, d));
     a.g[idx] = gv;
     if (a.grad_out) a.grad_out[(int64_t)c * a.D + d] = gv;
     plp += prior_logp(a, d, zv);
@@ -994,6 +1018,10 @@ static ChainArgs chain_args(edhmc_ctx* ctx) {
   a.z_lp = ctx->z_lp.p;
   a.logp = (double*)ctx->logp.p; a.K0 = (double*)ctx->K0.p;
   a.ploc = (const float*)ctx->ploc.p; a.pscale = (const float*)ctx->pscale.p;
+  if (ctx->has_kinds) {
+    a.pkind = (const uint8_t*)ctx->pkind.p;
+    a.plnorm = (const double*)ctx->plnorm.p;
+  }
   a.Gpart = (const float*)ctx->Gpart.p; a.ksplit = ctx->ksplit;
   a.ll = (const double*)ctx->ll.p;
   a.lik_weight = ctx->lik_weight;
@@ -1335,7 +1363,7 @@ edhmc_ctx::~edhmc_ctx() {
   if (hier) hier_destroy(hier);
   DevBuf* bufs[] = {&Xr, &Xc, &y, &ploc, &pscale, &z, &r, &g, &z0, &g0, &z_lp, &logp, &K0,
                     &flag, &ndiv, &Rt, &Gpart, &llpart, &ll, &mom_mean, &mom_m2, &tmp_count,
-                    &stats_part, &pred_col, &ll_split, &xty, &xty_part};
+                    &stats_part, &pred_col, &ll_split, &xty, &xty_part, &pkind, &plnorm};
   for (DevBuf* b : bufs) b->release();
 }
 
@@ -1392,6 +1420,7 @@ int edhmc_bind_glm(edhmc_ctx* ctx, const void* X, int32_t x_in_dtype, const floa
   ctx->lik = likelihood;
   ctx->lik_scale = lik_scale;
   ctx->lik_weight = 1.0f;
+  ctx->has_kinds = false;
   ctx->N = N;
   ctx->N_pad = round_up(N, kRowPad);
   ctx->D = D;
@@ -1731,9 +1760,59 @@ static int mh_step(edhmc_ctx* ctx, const float* proposal_scale, const float* bet
 
 int edhmc_set_lik_weight(edhmc_ctx* ctx, float w) {
   if (!ctx) return EDHMC_EINVAL;
-  if (!(w > 0.0f) || !std::isfinite(w)) return set_err(ctx, EDHMC_EINVAL, "weight must be > 0");
+  if (!(w >= 0.0f) || !std::isfinite(w)) return set_err(ctx, EDHMC_EINVAL, "weight must be >= 0");
   ctx->lik_weight = w;
   ctx->g_valid = false;
+  return EDHMC_OK;
+}
+
+__global__ void prior_family_kernel(const uint8_t* __restrict__ kind,
+                                    const double* __restrict__ lnorm, int D, int D_pad,
+                                    uint8_t* __restrict__ kind_out, double* __restrict__ lnorm_out,
+                                    int* bad) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= D_pad) return;
+  uint8_t k = 0;
+  double l = 0.0;
+  if (d < D) {
+    k = kind[d];
+    l = lnorm[d];
+    if ((k & 15) > kPriorInvGamma || (k >> 4) > kOpSigmoid || !isfinite(l)) atomicAdd(bad, 1);
+  }
+  kind_out[d] = k;
+  lnorm_out[d] = l;
+}
+
+int edhmc_set_prior_family(edhmc_ctx* ctx, const uint8_t* kind, const double* lnorm,
+                           void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  if (!ctx->bound) return set_err(ctx, EDHMC_ESTATE, "no model bound");
+  hipStream_t st = (hipStream_t)stream;
+  ctx->g_valid = false;
+  surrogate_invalidate(ctx->sur);
+  if (!kind) {
+    ctx->has_kinds = false;
+    return EDHMC_OK;
+  }
+  if (!lnorm) return set_err(ctx, EDHMC_EINVAL, "lnorm is NULL");
+  EDHMC_TRY(dev_alloc(ctx, ctx->pkind, (size_t)ctx->D_pad, false));
+  EDHMC_TRY(dev_alloc(ctx, ctx->plnorm, (size_t)ctx->D_pad * 8, false));
+  EDHMC_TRY(dev_alloc(ctx, ctx->tmp_count, 4));
+  int* bad = (int*)ctx->tmp_count.p;
+  hipLaunchKernelGGL(prior_family_kernel, dim3((ctx->D_pad + 255) / 256), dim3(256), 0, st, kind,
+                     lnorm, ctx->D, ctx->D_pad, (uint8_t*)ctx->pkind.p, (double*)ctx->plnorm.p,
+                     bad);
+  EDHMC_TRY(check_launch(ctx, "prior_family"));
+  int nbad = 0;
+  if (hipMemcpyAsync(&nbad, bad, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return set_err(ctx, EDHMC_EHIP, "set_prior_family: copy failed");
+  if (nbad) {
+    ctx->has_kinds = false;
+    return set_err(ctx, EDHMC_EINVAL, "unknown prior family/operand code or non-finite constant");
+  }
+  ctx->has_kinds = true;
   return EDHMC_OK;
 }
 

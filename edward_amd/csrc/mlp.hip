@@ -943,6 +943,9 @@ int edhmc_transition_surrogate(edhmc_ctx* ctx, float eps, int32_t n_steps, const
   if (!ctx) return EDHMC_EINVAL;
   DeviceGuard dg(ctx->device);
   EDHMC_TRY(check_sur(ctx));
+  if (ctx->has_kinds)
+    return set_err(ctx, EDHMC_EUNSUPPORTED,
+                   "the surrogate transition needs Normal priors on every coordinate");
   return sur_transition(ctx, eps, n_steps, r0, u, seed, counter, accepted_out, sample_out,
                         n_accept, (hipStream_t)stream);
 }

@@ -127,6 +127,21 @@ class GLMEngine(Context):
                                        self.stream))
     torch.cuda.synchronize(dev)
 
+  def set_prior_family(self, kind, lnorm=None):
+    """Per-coordinate constrained prior families and operand maps (kind
+    byte = family | operand << 4, lnorm = normalising constants, see
+    ``csrc/transform.h``); ``kind=None`` restores Normal priors."""
+    if kind is None:
+      self._prior_family = None
+      self.check(self.lib.edhmc_set_prior_family(self.h, None, None, self.stream))
+      return
+    k = _dev_tensor(np.asarray(kind, np.uint8).reshape(-1), torch.uint8, self.device)
+    ln = _dev_tensor(np.asarray(lnorm, np.float64).reshape(-1), torch.float64, self.device)
+    if k.numel() != self.D or ln.numel() != self.D:
+      raise ValueError('prior family needs %d coordinates' % self.D)
+    self._prior_family = (k, ln)
+    self.check(self.lib.edhmc_set_prior_family(self.h, L.ptr(k), L.ptr(ln), self.stream))
+
   def set_lik_weight(self, w):
     """Multiplier of the likelihood in the log joint (Edward's ``scale`` of
     the observed variable, used by the stochastic-gradient samplers)."""

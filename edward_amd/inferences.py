@@ -115,16 +115,7 @@ class Inference(object):
     elif not isinstance(scale, dict):
       raise TypeError('scale must be a dict object.')
     self.scale = scale
-    self.transformations = {}
-    if auto_transform:
-      self.latent_vars_unconstrained = {}
-      for z, qz in self.latent_vars.items():
-        if getattr(z, 'support', None) not in (None, 'real', 'multivariate_real') and \
-                getattr(qz, 'support', None) != 'point':
-          raise NotImplementedError(
-              'auto_transform of support %r is not implemented; only real-support '
-              'latents (identity transform) are.' % z.support)
-        self.latent_vars_unconstrained[z] = qz
+    self._auto_transform(auto_transform)
     self.logging = logdir is not None
     if self.logging:
       if log_timestamp:
@@ -149,6 +140,43 @@ class Inference(object):
     """``inference.py:334-338``."""
     if getattr(self, 'logging', False):
       pass
+
+  def _auto_transform(self, auto_transform):
+    """``inference.py:222-264``: with ``auto_transform``, a latent whose
+    support differs from its approximation's is paired with its unconstrained
+    version ``ed.transform(z)`` (``transformations``); an ``Empirical``
+    approximation (support ``'points'``) then holds the unconstrained values,
+    and ``latent_vars`` maps the latent to the pushed-forward view whose
+    ``params`` are the constrained samples."""
+    from edward_amd import bijectors
+    if not hasattr(self, '_user_latent_vars'):
+      self._user_latent_vars = dict(self.latent_vars)
+    latent_vars = self._user_latent_vars
+    self.transformations = {}
+    self._transforms = {}
+    if not auto_transform:
+      self.latent_vars = dict(latent_vars)
+      if hasattr(self, 'latent_vars_unconstrained'):
+        del self.latent_vars_unconstrained
+      return
+    self.latent_vars = {}
+    self.latent_vars_unconstrained = {}
+    for z, qz in latent_vars.items():
+      zs, qs = getattr(z, 'support', None), getattr(qz, 'support', None)
+      if zs is not None and qs is not None and zs != qs and qs != 'point':
+        z_u = bijectors.transform(z)
+        self.transformations[z] = z_u
+        qz_u = qz if qs == 'points' else bijectors.transform(qz)
+        self.latent_vars_unconstrained[z_u] = qz_u
+        if z_u is not z:
+          self._transforms[z] = z_u.bijector.bijector.name   # the map u -> theta
+          qz_c = M.ConstrainedEmpirical(qz_u, z_u.bijector)
+        else:
+          qz_c = qz_u
+        self.latent_vars[z] = qz_c
+      else:
+        self.latent_vars[z] = qz
+        self.latent_vars_unconstrained[z] = qz
 
   def _log(self, rec):
     if getattr(self, 'logging', False):
@@ -175,7 +203,7 @@ class MonteCarlo(Inference):
     super(MonteCarlo, self).__init__(latent_vars, data)
 
   def initialize(self, *args, **kwargs):
-    kwargs['n_iter'] = int(np.amin([qz.params.shape[0] for qz in self.latent_vars.values()]))
+    kwargs['n_iter'] = int(np.amin([qz.params.shape[0] for qz in self._stores().values()]))
     super(MonteCarlo, self).initialize(*args, **kwargs)
     self.train = self.build_update()
     self._reset_counts()
@@ -200,13 +228,30 @@ class MonteCarlo(Inference):
   # -- model binding (shared by the samplers) ------------------------------
   _scale_aware = False     # SGLD / SGHMC honour `scale`; HMC ignores it (hmc.py:161-192)
 
+  _constrained_ok = False  # only HMC samples auto-transformed latents (hmc.py:132-159)
+
+  def _stores(self):
+    """The user's ``Empirical`` stores (for an auto-transformed latent they
+    hold the unconstrained positions, as in the reference)."""
+    return getattr(self, '_user_latent_vars', self.latent_vars)
+
+  def _compile_transforms(self):
+    tr = getattr(self, '_transforms', {})
+    if tr and not self._constrained_ok:
+      raise NotImplementedError(
+          '%s cannot sample constrained latents: the reference writes its samples into '
+          'the derived constrained tensor, which is not a store; use HMC, or give the '
+          'latent unconstrained support' % type(self).__name__)
+    return tr
+
   def _prepare(self):
     """Chain count and device from the stores; compile the model now if
     every placeholder it needs is bound, else at the first fed update."""
     from edward_amd.engine import require_gpu
     require_gpu()
-    first = next(iter(self.latent_vars))
-    self.n_chains = _compile._chains_of(first, self.latent_vars[first])
+    stores = self._stores()
+    first = next(iter(stores))
+    self.n_chains = _compile._chains_of(first, stores[first])
     self._device = torch.device('cuda', 0)
     self.program = None
     self.engine = None
@@ -217,8 +262,9 @@ class MonteCarlo(Inference):
       pass
 
   def _compile(self, data):
-    prog = _compile.compile_model(self.latent_vars, data, precision=self.precision,
-                                  scale=self.scale if self._scale_aware else None)
+    prog = _compile.compile_model(self._stores(), data, precision=self.precision,
+                                  scale=self.scale if self._scale_aware else None,
+                                  transforms=self._compile_transforms())
     self.program = prog
     self.engine = prog.engine
     self.n_chains = prog.n_chains
@@ -237,8 +283,9 @@ class MonteCarlo(Inference):
     if self.program is None:
       self._compile(data)
     else:
-      _compile.rebind_model(self.program, self.latent_vars, data,
-                            self.scale if self._scale_aware else None)
+      _compile.rebind_model(self.program, self._stores(), data,
+                            self.scale if self._scale_aware else None,
+                            self._compile_transforms())
     self._state_t = None
 
   def _check_bound(self):
@@ -296,6 +343,8 @@ class HMC(MonteCarlo):
     step_size_jitter  j in [0, 1): transition t uses step_size * U(1-j, 1+j)
                (default 0: the reference's fixed step size)
   """
+
+  _constrained_ok = True
 
   def __init__(self, *args, **kwargs):
     super(HMC, self).__init__(*args, **kwargs)

@@ -210,6 +210,105 @@ class Bernoulli(RandomVariable):
     self.batch_shape = s if s is not None else ()
 
 
+class _Parametric(RandomVariable):
+  """A distribution given by named parameters (constants or nodes); batch
+  shape is their broadcast shape, dtype the first floating parameter's."""
+
+  _params = ()
+
+  def __init__(self, *args, **kwargs):
+    names = self._params
+    if len(args) > len(names):
+      raise TypeError('%s takes %d parameters' % (type(self).__name__, len(names)))
+    vals = dict(zip(names, args))
+    for n in names:
+      if n in kwargs:
+        vals[n] = kwargs.pop(n)
+      if n not in vals:
+        raise TypeError('%s needs the parameter %r' % (type(self).__name__, n))
+    dtype = kwargs.pop('dtype', None)
+    ss = kwargs.pop('sample_shape', ())
+    name = kwargs.pop('name', None)
+    for k in ('validate_args', 'allow_nan_stats'):
+      kwargs.pop(k, None)
+    if kwargs:
+      raise TypeError('unexpected arguments %s' % sorted(kwargs))
+    super(_Parametric, self).__init__(sample_shape=ss, name=name, **vals)
+    for n in names:
+      setattr(self, n, vals[n])
+    dts = [d for d in (_param_dtype(vals[n]) for n in names) if d]
+    self.dtype = dtype or (dts[0] if dts else 'float32')
+    shapes = [sh for sh in (_param_shape(vals[n]) for n in names) if sh is not None]
+    self.batch_shape = tuple(np.broadcast_shapes(*shapes)) if shapes else ()
+
+  def param_values(self):
+    """The parameters as float64 arrays of the batch shape (constants only)."""
+    out = []
+    for n in self._params:
+      v = getattr(self, n)
+      if isinstance(v, Node):
+        raise NotImplementedError('parameter %s of %s must be a constant' % (n, self.name))
+      out.append(np.broadcast_to(_as_array(v).astype(np.float64), self.batch_shape))
+    return out
+
+
+class Gamma(_Parametric):
+  """Gamma(concentration, rate); ``support = 'nonnegative'``
+  (``random_variables.py:37``)."""
+
+  support = 'nonnegative'
+  _params = ('concentration', 'rate')
+
+
+class Exponential(_Parametric):
+  """Exponential(rate) = Gamma(1, rate); ``support = 'nonnegative'``
+  (``random_variables.py:36``)."""
+
+  support = 'nonnegative'
+  _params = ('rate',)
+
+
+class Chi2(_Parametric):
+  """Chi2(df) = Gamma(df / 2, 1 / 2); ``support = 'nonnegative'``
+  (``random_variables.py:34``)."""
+
+  support = 'nonnegative'
+  _params = ('df',)
+
+
+class InverseGamma(_Parametric):
+  """InverseGamma(concentration, rate); ``support = 'nonnegative'``
+  (``random_variables.py:38``)."""
+
+  support = 'nonnegative'
+  _params = ('concentration', 'rate')
+
+
+class Beta(_Parametric):
+  """Beta(concentration1, concentration0); ``support = '01'``
+  (``random_variables.py:31``)."""
+
+  support = '01'
+  _params = ('concentration1', 'concentration0')
+
+
+class TransformedDistribution(RandomVariable):
+  """``TransformedDistribution(distribution, bijector)``: the law of
+  ``bijector.forward(x)`` for ``x ~ distribution``.  It has no ``support``
+  unless one is assigned (as Edward's generated class, which is not in the
+  support table of ``random_variables.py:29-43``; ``util/random_variables.py:
+  856-917`` sets it on the results of ``transform``)."""
+
+  def __init__(self, distribution, bijector, name=None, sample_shape=(), **kw):
+    super(TransformedDistribution, self).__init__(distribution=distribution, bijector=bijector,
+                                                  name=name, sample_shape=sample_shape)
+    self.distribution, self.bijector = distribution, bijector
+    self.dtype = getattr(distribution, 'dtype', 'float32')
+    base = tuple(getattr(distribution, 'batch_shape', ())) + \
+        tuple(getattr(distribution, 'event_shape', ()))
+    self.batch_shape = tuple(bijector.forward_event_shape(base))
+
+
 class Empirical(RandomVariable):
   """Sample store ``Empirical(params[T, ...])`` (``edward/models/empirical.py``).
 
@@ -278,6 +377,46 @@ class Empirical(RandomVariable):
       g.manual_seed(int(seed))
     idx = torch.randint(0, self.n, (int(n),), generator=g).to(self._params.device)
     return self._params.index_select(0, idx)
+
+
+class ConstrainedEmpirical(TransformedDistribution):
+  """An ``Empirical`` of unconstrained samples seen through the inverse of an
+  auto-transform (``inference.py:240-255``): ``params`` are the constrained
+  samples ``bijector.forward(qz.params)``, recomputed from the store on every
+  access, with the store's ``mean``/``variance``/``stddev``/``sample`` over
+  them."""
+
+  def __init__(self, qz_unconstrained, z_bijector, name=None):
+    from edward_amd import bijectors
+    super(ConstrainedEmpirical, self).__init__(qz_unconstrained,
+                                               bijectors.Invert(z_bijector), name=name)
+    self.batch_shape = ()
+
+  @property
+  def params(self):
+    p = self.distribution.params
+    return self.bijector.forward(p).to(p.dtype)
+
+  @property
+  def n(self):
+    return self.distribution.n
+
+  @property
+  def event_shape(self):
+    return self.distribution.event_shape
+
+  def mean(self, burn_in=0):
+    return self.params[burn_in:].double().mean(0).to(self.distribution.params.dtype)
+
+  def variance(self, burn_in=0):
+    p = self.params[burn_in:].double()
+    return ((p - p.mean(0)) ** 2).mean(0).to(self.distribution.params.dtype)
+
+  def stddev(self, burn_in=0):
+    return torch.sqrt(self.variance(burn_in))
+
+  def sample(self, n=1, seed=None):
+    return self.bijector.forward(self.distribution.sample(n, seed))
 
 
 def reset_graph():

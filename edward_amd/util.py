@@ -36,3 +36,9 @@ def set_seed(x):
 
 def get_seed():
   return 0 if _SEED[0] is None else _SEED[0]
+
+
+def transform(x, *args, **kwargs):
+  """``ed.util.transform`` (``util/random_variables.py:856-917``)."""
+  from edward_amd.bijectors import transform as _transform
+  return _transform(x, *args, **kwargs)

@@ -183,8 +183,27 @@ int edhmc_replica_exchange_step(edhmc_ctx* ctx, const float* proposal_scale, con
 /* Multiplier of the likelihood term in the log joint and its gradient:
  * Edward's `scale` of the observed variable (inference.py:187-193, used by
  * sgld.py:95-121 and sghmc.py:99-135 for minibatches; HMC ignores it,
- * hmc.py:161-192, so the HMC entry points see 1).  bind resets it to 1. */
+ * hmc.py:161-192, so the HMC entry points see 1; a model without observed
+ * data binds one zero row with weight 0).  bind resets it to 1. */
 int edhmc_set_lik_weight(edhmc_ctx* ctx, float weight);
+
+/* Constrained priors for auto-transformed latents (edward/inferences/
+ * inference.py:222-264 builds the transformations; hmc.py:132-159 adds the
+ * log-Jacobian of the map from the unconstrained position u).  kind [D]
+ * uint8 (device): low nibble = family of the prior in u-space, high nibble =
+ * the operand the linear predictor receives:
+ *   family 0 Normal(loc, scale) on u           operand 0 u
+ *          1 Gamma(loc=a, scale=b) at softplus(u)       1 softplus(u)
+ *          2 Beta(loc=a, scale=b) at sigmoid(u)         2 sigmoid(u)
+ *          3 InverseGamma(loc=a, scale=b) at softplus(u)
+ * where loc/scale are the bound prior vectors, read as the family's two
+ * parameters; lnorm [D] fp64 (device) is each coordinate's normalising
+ * constant (Normal: -log scale - log(2 pi)/2; Gamma and InverseGamma:
+ * a log b - lgamma(a); Beta: -log B(a, b)).  kind = NULL restores plain
+ * Normal priors with identity operands.  Binding a model clears it.  Not
+ * available with the surrogate transition (EDHMC_EUNSUPPORTED there). */
+int edhmc_set_prior_family(edhmc_ctx* ctx, const uint8_t* kind, const double* lnorm,
+                           void* stream);
 
 /* The run loop of Inference.run (edward/inferences/inference.py:145-147):
  * transitions t = t0 .. t0+n_iter-1, each writing params[t] of the sample

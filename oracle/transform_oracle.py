@@ -1,0 +1,174 @@
+"""FP64 CPU restatement of Edward's auto-transform for HMC -- TEST
+INFRASTRUCTURE ONLY (imported by tests/ and nothing in edward_amd/).
+
+Follows:
+  * ``edward/util/random_variables.py:856-917`` (``transform``): support
+    '01' -> inverse sigmoid, 'nonnegative' -> inverse softplus, 'simplex' ->
+    inverse softmax-centered, real -> identity.
+  * ``edward/inferences/inference.py:222-264``: the unconstrained latent is
+    paired with the Empirical store, which holds the unconstrained values.
+  * ``edward/inferences/hmc.py:132-159`` (``_log_joint_unconstrained``):
+    theta = bijector.inverse(u) and the log joint gains
+    sum inverse_log_det_jacobian, i.e. log p(x, theta(u)) + log |dtheta/du|.
+The constrained log densities are the textbook ones (Gamma(concentration,
+rate), InverseGamma(concentration, rate), Beta(concentration1,
+concentration0)); pinned in tests/test_transform_oracle.py by scipy.stats,
+quadrature (the u-density integrates to one and reproduces the constrained
+moments), finite differences and the Beta-Bernoulli conjugate posterior.
+"""
+
+import math
+
+import numpy as np
+
+from oracle.hmc_oracle import GLM, sigmoid, softplus
+
+FAMILY_NORMAL, FAMILY_GAMMA, FAMILY_BETA, FAMILY_INVGAMMA = 0, 1, 2, 3
+OP_IDENTITY, OP_SOFTPLUS, OP_SIGMOID = 0, 1, 2
+
+_lgamma = np.vectorize(math.lgamma, otypes=[np.float64])
+
+
+def log_sigmoid(u):
+  return -softplus(-u)
+
+
+def constrained_logpdf(family, theta, p1, p2):
+  """log density of the constrained prior at theta."""
+  theta = np.asarray(theta, np.float64)
+  if family == FAMILY_GAMMA:
+    return p1 * np.log(p2) - _lgamma(p1) + (p1 - 1) * np.log(theta) - p2 * theta
+  if family == FAMILY_INVGAMMA:
+    return p1 * np.log(p2) - _lgamma(p1) - (p1 + 1) * np.log(theta) - p2 / theta
+  if family == FAMILY_BETA:
+    return ((p1 - 1) * np.log(theta) + (p2 - 1) * np.log1p(-theta)
+            + _lgamma(p1 + p2) - _lgamma(p1) - _lgamma(p2))
+  raise ValueError(family)
+
+
+def to_constrained(family, u):
+  """theta = bijector.inverse(u) of the family's support."""
+  return sigmoid(u) if family == FAMILY_BETA else softplus(u)
+
+
+def log_jacobian(family, u):
+  """log |dtheta/du| (the inverse_log_det_jacobian term of hmc.py:153-154)."""
+  if family == FAMILY_BETA:
+    return log_sigmoid(u) + log_sigmoid(-u)
+  return log_sigmoid(u)
+
+
+def prior_logp_grad(u, family, p1, p2):
+  """(log p_u(u), d/du) for one coordinate family, elementwise in FP64.
+  Normal: the density of u itself (a real-support latent, or a Normal pushed
+  through the bijector of its support, whose Jacobians cancel)."""
+  u = np.asarray(u, np.float64)
+  if family == FAMILY_NORMAL:
+    z = (u - p1) / p2
+    return -0.5 * z * z - np.log(p2) - 0.5 * math.log(2 * math.pi), -z / p2
+  th = to_constrained(family, u)
+  if family == FAMILY_BETA:
+    # log theta and log(1 - theta) from u, so that saturated sigmoids stay finite
+    lp = ((p1 - 1) * log_sigmoid(u) + (p2 - 1) * log_sigmoid(-u)
+          + _lgamma(p1 + p2) - _lgamma(p1) - _lgamma(p2))
+  else:
+    lp = constrained_logpdf(family, th, p1, p2)
+  lp = lp + log_jacobian(family, u)
+  s = sigmoid(u)
+  if family == FAMILY_GAMMA:
+    d = ((p1 - 1) / th - p2) * s + (1 - s)
+  elif family == FAMILY_INVGAMMA:
+    d = (-(p1 + 1) / th + p2 / th ** 2) * s + (1 - s)
+  else:
+    # dtheta/du = s (1 - s): d/du [(a-1) log s + (b-1) log(1-s)] + d log(s(1-s))/du
+    d = (p1 - 1) * (1 - s) - (p2 - 1) * s + (1 - 2 * s)
+  return lp, d
+
+
+def operand(u, op):
+  if op == OP_SOFTPLUS:
+    return softplus(u)
+  if op == OP_SIGMOID:
+    return sigmoid(u)
+  return np.asarray(u, np.float64)
+
+
+def d_operand(u, op):
+  if op == OP_SOFTPLUS:
+    return sigmoid(u)
+  if op == OP_SIGMOID:
+    s = sigmoid(u)
+    return s * (1 - s)
+  return np.ones_like(np.asarray(u, np.float64))
+
+
+class TransformedGLM(object):
+  """Log joint in the unconstrained space of ``y ~ Lik(dot(X, m(u)))``
+  with per-coordinate prior families (``hmc.py:132-159`` over the GLM of
+  ``hmc.py:161-192``): m(u) is the operand map of each coordinate.
+  ``lik_weight`` multiplies the likelihood (0 for a model with no data)."""
+
+  def __init__(self, X, y, family, op, p1, p2, likelihood='bernoulli', lik_scale=1.0,
+               lik_weight=1.0):
+    self.glm = GLM(X, y, 0.0, 1.0, likelihood, lik_scale)
+    D = self.glm.D
+    self.family = np.broadcast_to(np.asarray(family), (D,))
+    self.op = np.broadcast_to(np.asarray(op), (D,))
+    self.p1 = np.broadcast_to(np.asarray(p1, np.float64), (D,))
+    self.p2 = np.broadcast_to(np.asarray(p2, np.float64), (D,))
+    self.lik_weight = float(lik_weight)
+
+  def logp_grad(self, U):
+    U = np.atleast_2d(np.asarray(U, np.float64))
+    C, D = U.shape
+    M = np.empty_like(U)
+    dM = np.empty_like(U)
+    lp = np.zeros(C)
+    gp = np.empty_like(U)
+    for d in range(D):
+      M[:, d] = operand(U[:, d], self.op[d])
+      dM[:, d] = d_operand(U[:, d], self.op[d])
+      l, g = prior_logp_grad(U[:, d], self.family[d], self.p1[d], self.p2[d])
+      lp += l
+      gp[:, d] = g
+    eta = self.glm.eta(M)
+    ll = self.glm.loglik_terms(eta).sum(axis=1)
+    gl = self.glm.dloglik_deta(eta) @ self.glm.X
+    return self.lik_weight * ll + lp, self.lik_weight * gl * dM + gp
+
+  def logp(self, U):
+    return self.logp_grad(U)[0]
+
+
+# ----------------------------------------------------------------------------
+# Bijector maps (tf.contrib.distributions.bijectors semantics)
+# ----------------------------------------------------------------------------
+
+
+def softplus_inverse(y):
+  y = np.asarray(y, np.float64)
+  return y + np.log(-np.expm1(-y))
+
+
+def logit(y):
+  y = np.asarray(y, np.float64)
+  return np.log(y) - np.log1p(-y)
+
+
+def softmax_centered_forward(x):
+  """R^K -> simplex in R^(K+1): softmax of [x, 0]."""
+  x = np.asarray(x, np.float64)
+  z = np.concatenate([x, np.zeros(x.shape[:-1] + (1,))], axis=-1)
+  z = z - z.max(axis=-1, keepdims=True)
+  e = np.exp(z)
+  return e / e.sum(axis=-1, keepdims=True)
+
+
+def softmax_centered_inverse(y):
+  y = np.asarray(y, np.float64)
+  return np.log(y[..., :-1]) - np.log(y[..., -1:])
+
+
+def softmax_centered_fldj(x):
+  """log |det d y_{1..K} / dx| = sum_{i=1}^{K+1} log y_i."""
+  return np.log(softmax_centered_forward(x)).sum(axis=-1)
