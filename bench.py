@@ -63,6 +63,12 @@ def parse():
   ap.add_argument('--rec-start-tol', type=float, default=300.0,
                   help='pair recording starts when the mean log joint moves less than this '
                        'over 5 burn-in transitions')
+  ap.add_argument('--init', default='laplace', choices=['laplace', 'prior'],
+                  help='chain start: draws from the Laplace approximation N(mode, P^-1) found '
+                       'by device Newton steps (timed as warm-up), or the workload\'s N(0, I) '
+                       'draws')
+  ap.add_argument('--laplace-burn-in', type=int, default=2,
+                  help='transitions before pair recording starts after a Laplace start')
   ap.add_argument('--burn-in-adapt', type=int, default=1,
                   help='1: the burn-in step-size centre follows the acceptance rate')
   ap.add_argument('--plain-steps', type=int, default=8, help='timed plain fp32 transitions')
@@ -275,9 +281,26 @@ def main():
   bf.set_state(wl.W0)
   torch.cuda.synchronize()
 
-  # timed region: burn-in + pair recording + training + K sampling transitions
-  # 1. burn-in with plain bf16 HMC, then record (w, grad) pairs
+  # timed region: warm start + burn-in + pair recording + training + K sampling
+  # transitions
+  # 0. warm start: the Laplace approximation by device Newton steps, chains
+  #    drawn from N(mode, P^-1)
   t_warm0 = time.time()
+  burn_in = args.burn_in
+  laplace_info = None
+  if args.init == 'laplace':
+    import scipy.linalg
+    mode, P, lp_mode, n_newton = bf.laplace()
+    chol = np.linalg.cholesky(P)
+    e = np.random.default_rng(args.seed + 3).standard_normal((D, C))
+    W_start = mode[None, :] + scipy.linalg.solve_triangular(chol, e, lower=True, trans='T').T
+    bf.set_state(W_start.astype(np.float32))
+    burn_in = args.laplace_burn_in
+    laplace_info = {'newton_iterations': n_newton, 'seconds': time.time() - t_warm0,
+                    'logp_mode': lp_mode}
+    log('laplace start: %d Newton steps, %.2f s, logp(mode) %.2f'
+        % (n_newton, laplace_info['seconds'], lp_mode))
+  # 1. burn-in with plain bf16 HMC, then record (w, grad) pairs
   means = []
   # training pairs: every rec_stride-th inner leapfrog point of n_rec
   # transitions (~250k pairs from more independent trajectories than a
@@ -292,14 +315,14 @@ def main():
   jit = np.random.default_rng(args.seed + 7)
   log_c = math.log(eps)
   recording, n_recorded = False, 0
-  while t < args.burn_in + n_rec:
+  while t < burn_in + n_rec:
     # Pair recording overlaps the end of the burn-in: it starts once the mean
     # log joint moves by less than --rec-start-tol over 5 transitions (near
     # equilibrium), or after --burn-in transitions, and records n_rec
     # transitions; the burn-in ends when the mean moves by less than 20 over 5
     # transitions (or --burn-in + n_rec transitions ran) and the recording is
     # full.
-    if not recording and (t >= args.burn_in or
+    if not recording and (t >= burn_in or
                           (t >= 10 and abs(means[-1] - means[-6]) < args.rec_start_tol)):
       bf.record_pairs(Wrec, Grec, stride=rec_stride)
       recording = True
@@ -329,7 +352,7 @@ def main():
       n_recorded += 1
     if converged_at is None and t >= 10 and abs(means[-1] - means[-6]) < 20.0:
       converged_at = t
-    if n_recorded >= n_rec and (converged_at is not None or t >= args.burn_in + n_rec):
+    if n_recorded >= n_rec and (converged_at is not None or t >= burn_in + n_rec):
       break
   bf.record_pairs(None)
   torch.cuda.synchronize()
@@ -449,9 +472,11 @@ def main():
       'steps': K,
       'warmup': args.warmup,
       'ms_per_step': incl_s / K * 1e3,
-      'timed_region': 'burn-in (%d plain bf16 transitions, pairs recorded over the last %d) + '
+      'timed_region': '%sburn-in (%d plain bf16 transitions, pairs recorded over the last %d) + '
                       'surrogate training + %d surrogate transitions: value = min-ESS of the %d '
-                      'samples / all of it' % (n_warm, n_rec, K, K),
+                      'samples / all of it' % ('Laplace start (device Newton steps) + '
+                                               if laplace_info else '', n_warm, n_rec, K, K),
+      'warm_start': laplace_info if laplace_info else 'N(0, I) draws',
       'higher_is_better': True,
       'scaling': 'weak',
       'vs_baseline': None,

@@ -99,6 +99,7 @@ struct edhmc_ctx {
   edhmc::DevBuf pred_col;   // posterior-predictive column partials [m_tiles][N_pad]
   edhmc::DevBuf ll_split;   // log-likelihood range sums [16][C_pad] (fixed-order reduction)
   edhmc::DevBuf xty, xty_part;   // X^T y (FP64, Bernoulli) and its row-chunk partials
+  edhmc::DevBuf newton;          // scratch of edhmc_newton_terms
 
   edhmc::Profiler prof;
   edhmc::Surrogate* sur = nullptr;

@@ -1363,7 +1363,8 @@ edhmc_ctx::~edhmc_ctx() {
   if (hier) hier_destroy(hier);
   DevBuf* bufs[] = {&Xr, &Xc, &y, &ploc, &pscale, &z, &r, &g, &z0, &g0, &z_lp, &logp, &K0,
                     &flag, &ndiv, &Rt, &Gpart, &llpart, &ll, &mom_mean, &mom_m2, &tmp_count,
-                    &stats_part, &pred_col, &ll_split, &xty, &xty_part, &pkind, &plnorm};
+                    &stats_part, &pred_col, &ll_split, &xty, &xty_part, &pkind, &plnorm,
+                    &newton};
   for (DevBuf* b : bufs) b->release();
 }
 

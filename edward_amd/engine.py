@@ -142,6 +142,45 @@ class GLMEngine(Context):
     self._prior_family = (k, ln)
     self.check(self.lib.edhmc_set_prior_family(self.h, L.ptr(k), L.ptr(ln), self.stream))
 
+  def newton_terms(self, w):
+    """(log p, gradient [D], negative Hessian [D, D]) of the log joint at one
+    position w [D], in float64 (``edhmc_newton_terms``)."""
+    w = _dev_tensor(np.asarray(w.cpu() if isinstance(w, torch.Tensor) else w,
+                               np.float32).reshape(-1), torch.float32, self.device)
+    if w.numel() != self.D:
+      raise ValueError('position has %d coordinates, model has %d' % (w.numel(), self.D))
+    lp = torch.empty(1, dtype=torch.float64, device=self.device)
+    g = torch.empty(self.D, dtype=torch.float64, device=self.device)
+    P = torch.empty(self.D, self.D, dtype=torch.float64, device=self.device)
+    self.check(self.lib.edhmc_newton_terms(self.h, L.ptr(w), L.ptr(lp), L.ptr(g), L.ptr(P),
+                                           self.stream))
+    return float(lp.item()), g.cpu().numpy(), P.cpu().numpy()
+
+  def laplace(self, w0=None, max_iter=30, tol=1e-6):
+    """Mode of the log joint by damped Newton steps from w0 (default 0) and
+    the precision matrix there: the Laplace approximation N(mode, P^-1)
+    (``edward/inferences/laplace.py:17-137`` fits the same Gaussian).  A
+    step is halved until the log joint does not decrease; the loop stops
+    when the Newton decrement g^T P^-1 g / 2 falls below ``tol``.  Returns
+    (mode, P, logp at the mode, iterations)."""
+    w = np.zeros(self.D) if w0 is None else np.asarray(w0, np.float64).reshape(-1).copy()
+    lp, g, P = self.newton_terms(w)
+    it = 0
+    for it in range(1, max_iter + 1):
+      step = np.linalg.solve(P, g)
+      dec = 0.5 * float(g @ step)
+      if dec < tol:
+        break
+      t = 1.0
+      while True:
+        wn = w + t * step
+        lpn, gn, Pn = self.newton_terms(wn)
+        if lpn >= lp - 1e-9 * abs(lp) or t < 1e-6:
+          break
+        t *= 0.5
+      w, lp, g, P = wn, lpn, gn, Pn
+    return w, P, lp, it
+
   def set_lik_weight(self, w):
     """Multiplier of the likelihood in the log joint (Edward's ``scale`` of
     the observed variable, used by the stochastic-gradient samplers)."""

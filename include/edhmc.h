@@ -205,6 +205,16 @@ int edhmc_set_lik_weight(edhmc_ctx* ctx, float weight);
 int edhmc_set_prior_family(edhmc_ctx* ctx, const uint8_t* kind, const double* lnorm,
                            void* stream);
 
+/* Newton terms of the bound GLM's log joint at one position w [D] (fp32,
+ * device): logp_out (fp64 scalar), grad_out [D] fp64 and neg_hess_out
+ * [D, D] fp64 = -Hessian = X^T diag(s) X + diag(1 / scale^2), s_n the
+ * likelihood curvature in eta_n.  The mode and N(mode, neg_hess^-1) are
+ * the Laplace approximation (edward/inferences/laplace.py:17-137); the
+ * engine uses them for a warm start of the chains.  Normal priors and
+ * likelihood weight 1 only (EDHMC_EUNSUPPORTED otherwise). */
+int edhmc_newton_terms(edhmc_ctx* ctx, const float* w, double* logp_out, double* grad_out,
+                       double* neg_hess_out, void* stream);
+
 /* The run loop of Inference.run (edward/inferences/inference.py:145-147):
  * transitions t = t0 .. t0+n_iter-1, each writing params[t] of the sample
  * store params [T, C, D] and using momentum counter counter0 + t.  The chain

@@ -146,6 +146,18 @@ class GLM(object):
   def logp(self, W):
     return self.logp_grad(W)[0]
 
+  def neg_hessian(self, w):
+    """-d^2 logp / dw^2 at one position w [D]: X^T diag(s) X + diag(1 /
+    scale^2), s the likelihood's curvature in eta (sigma (1 - sigma) for
+    Bernoulli, 1 / lik_scale^2 for Normal)."""
+    eta = self.X @ np.asarray(w, np.float64)
+    if self.likelihood == 'bernoulli':
+      p = sigmoid(eta)
+      s = p * (1 - p)
+    else:
+      s = np.full(self.N, 1.0 / self.lik_scale ** 2)
+    return (self.X * s[:, None]).T @ self.X + np.diag(1.0 / self.prior_scale ** 2)
+
   def loglik(self, W):
     """Likelihood part only (no prior), [C]."""
     return self.loglik_terms(self.eta(W)).sum(axis=1)

@@ -437,3 +437,25 @@ def test_replica_exchange_keeps_the_tempered_targets():
   assert abs(z[:L].var() - 1.0) < 0.08
   assert abs(z[L:].var() - 4.0) < 0.35
   assert swaps > 0
+
+
+@pytest.mark.parametrize('likelihood', ['bernoulli', 'normal'])
+def test_glm_neg_hessian_is_minus_the_gradient_jacobian(likelihood):
+  """The Newton terms of the warm start: -Hessian by central differences of
+  the oracle's gradient; symmetric positive definite."""
+  rng = np.random.default_rng(21)
+  N, D = 60, 5
+  X = rng.normal(size=(N, D))
+  y = (rng.random(N) < 0.5).astype(np.float64) if likelihood == 'bernoulli' else rng.normal(size=N)
+  m = O.GLM(X, y, rng.normal(size=D) * 0.1, rng.uniform(0.5, 2.0, size=D), likelihood, 0.7)
+  w = rng.normal(size=D) * 0.3
+  P = m.neg_hessian(w)
+  h = 1e-6
+  J = np.empty((D, D))
+  for j in range(D):
+    e = np.zeros(D)
+    e[j] = h
+    J[:, j] = (m.logp_grad(w + e)[1][0] - m.logp_grad(w - e)[1][0]) / (2 * h)
+  np.testing.assert_allclose(P, -J, rtol=1e-6, atol=1e-6)
+  np.testing.assert_allclose(P, P.T, atol=1e-12)
+  assert np.linalg.eigvalsh(P).min() > 0
