@@ -68,7 +68,14 @@ def parse():
                        'by device Newton steps (timed as warm-up), or the workload\'s N(0, I) '
                        'draws')
   ap.add_argument('--laplace-burn-in', type=int, default=2,
-                  help='transitions before pair recording starts after a Laplace start')
+                  help='transitions before pair recording starts after a Laplace start '
+                       '(with --pairs trajectory)')
+  ap.add_argument('--pairs', default='laplace', choices=['laplace', 'trajectory'],
+                  help='surrogate training pairs: exact gradients at Laplace draws (with '
+                       '--init laplace; no burn-in transitions), or recorded along burn-in '
+                       'trajectories')
+  ap.add_argument('--n-pairs', type=int, default=229376,
+                  help='training pairs drawn from the Laplace approximation')
   ap.add_argument('--burn-in-adapt', type=int, default=1,
                   help='1: the burn-in step-size centre follows the acceptance rate')
   ap.add_argument('--plain-steps', type=int, default=8, help='timed plain fp32 transitions')
@@ -288,18 +295,22 @@ def main():
   t_warm0 = time.time()
   burn_in = args.burn_in
   laplace_info = None
+  pairs_from_laplace = args.init == 'laplace' and args.pairs == 'laplace'
   if args.init == 'laplace':
-    import scipy.linalg
-    mode, P, lp_mode, n_newton = bf.laplace()
-    chol = np.linalg.cholesky(P)
-    e = np.random.default_rng(args.seed + 3).standard_normal((D, C))
-    W_start = mode[None, :] + scipy.linalg.solve_triangular(chol, e, lower=True, trans='T').T
-    bf.set_state(W_start.astype(np.float32))
-    burn_in = args.laplace_burn_in
-    laplace_info = {'newton_iterations': n_newton, 'seconds': time.time() - t_warm0,
-                    'logp_mode': lp_mode}
+    from edward_amd.warmstart import LaplaceApproximation
+    lap = LaplaceApproximation(bf)
+    laplace_info = {'newton_iterations': lap.iterations, 'seconds': time.time() - t_warm0,
+                    'logp_mode': lap.logp_mode}
     log('laplace start: %d Newton steps, %.2f s, logp(mode) %.2f'
-        % (n_newton, laplace_info['seconds'], lp_mode))
+        % (lap.iterations, laplace_info['seconds'], lap.logp_mode))
+    if pairs_from_laplace:
+      Wrec, Grec = lap.gradient_pairs(args.n_pairs, C, seed=args.seed + 5)
+      torch.cuda.synchronize()
+      laplace_info['pairs_seconds'] = time.time() - t_warm0 - laplace_info['seconds']
+      log('training pairs: %d exact gradients at Laplace draws, %.2f s'
+          % (args.n_pairs, laplace_info['pairs_seconds']))
+    bf.set_state(lap.sample(C, seed=args.seed + 3))
+    burn_in = args.laplace_burn_in
   # 1. burn-in with plain bf16 HMC, then record (w, grad) pairs
   means = []
   # training pairs: every rec_stride-th inner leapfrog point of n_rec
@@ -307,15 +318,16 @@ def main():
   # dense record of fewer transitions)
   rec_stride = args.rec_stride
   n_rec = args.rec_transitions
-  cap = ((L - 1) // rec_stride) * C * n_rec
-  Wrec = torch.empty(cap, D, device=dev)
-  Grec = torch.empty(cap, D, device=dev)
+  if not pairs_from_laplace:
+    cap = ((L - 1) // rec_stride) * C * n_rec
+    Wrec = torch.empty(cap, D, device=dev)
+    Grec = torch.empty(cap, D, device=dev)
   t = 0
   converged_at = None
   jit = np.random.default_rng(args.seed + 7)
   log_c = math.log(eps)
   recording, n_recorded = False, 0
-  while t < burn_in + n_rec:
+  while not pairs_from_laplace and t < burn_in + n_rec:
     # Pair recording overlaps the end of the burn-in: it starts once the mean
     # log joint moves by less than --rec-start-tol over 5 transitions (near
     # equilibrium), or after --burn-in transitions, and records n_rec
@@ -354,9 +366,12 @@ def main():
       converged_at = t
     if n_recorded >= n_rec and (converged_at is not None or t >= burn_in + n_rec):
       break
-  bf.record_pairs(None)
   torch.cuda.synchronize()
-  n_pairs = bf.record_count()
+  if pairs_from_laplace:
+    n_pairs, n_rec = args.n_pairs, 0
+  else:
+    bf.record_pairs(None)
+    n_pairs = bf.record_count()
   n_warm = t
   warm_s = time.time() - t_warm0
   log('warm-up done: %d transitions, %d pairs, %.1f s' % (t, n_pairs, warm_s))
@@ -472,10 +487,14 @@ def main():
       'steps': K,
       'warmup': args.warmup,
       'ms_per_step': incl_s / K * 1e3,
-      'timed_region': '%sburn-in (%d plain bf16 transitions, pairs recorded over the last %d) + '
-                      'surrogate training + %d surrogate transitions: value = min-ESS of the %d '
-                      'samples / all of it' % ('Laplace start (device Newton steps) + '
-                                               if laplace_info else '', n_warm, n_rec, K, K),
+      'timed_region': ('Laplace start (device Newton steps, chains drawn from N(mode, P^-1)) '
+                       '+ %d exact gradients at Laplace draws as training pairs' % n_pairs
+                       if pairs_from_laplace else
+                       '%sburn-in (%d plain bf16 transitions, pairs recorded over the last %d)'
+                       % ('Laplace start (device Newton steps) + ' if laplace_info else '',
+                          n_warm, n_rec)) +
+                      ' + surrogate training + %d surrogate transitions: value = min-ESS of the '
+                      '%d samples / all of it' % (K, K),
       'warm_start': laplace_info if laplace_info else 'N(0, I) draws',
       'higher_is_better': True,
       'scaling': 'weak',

@@ -62,3 +62,21 @@ def test_laplace_mode_matches_oracle():
   # the gradient vanishes at the mode
   _, g, _ = eng.newton_terms(mode)
   assert np.abs(g).max() < 1e-3 * np.sqrt(np.diag(P)).max()
+
+
+def test_laplace_draws_and_gradient_pairs():
+  """Draws of N(mode, P^-1) have its mean and covariance (within Monte Carlo
+  error); the pairs hold the oracle's gradient at each draw."""
+  from edward_amd.warmstart import LaplaceApproximation
+  eng, m, _ = _case(4, 3000, 12)
+  lap = LaplaceApproximation(eng)
+  n = 40000
+  W = lap.sample(n, seed=1).cpu().numpy().astype(np.float64)
+  cov = np.linalg.inv(lap.precision)
+  sd = np.sqrt(np.diag(cov))
+  assert np.all(np.abs(W.mean(0) - lap.mode) < 5 * sd / np.sqrt(n))
+  corr_err = (np.cov(W.T) - cov) / np.outer(sd, sd)
+  assert np.abs(corr_err).max() < 5 * np.sqrt(2.0 / n)
+  Wp, Gp = lap.gradient_pairs(1000, 256, seed=2)
+  _, g_ref = m.logp_grad(Wp.cpu().numpy().astype(np.float64))
+  assert np.abs(Gp.cpu().numpy() - g_ref).max() <= 1e-4 * np.abs(g_ref).max()
