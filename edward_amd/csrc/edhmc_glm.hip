@@ -1145,9 +1145,203 @@ int glm_grad_eval_offset(edhmc_ctx* ctx, const float* goff, int64_t goff_ld, con
 // Buffers the chain-level engines above the GLM need to size their state.
 int glm_ensure_chains(edhmc_ctx* ctx, int C);
 
+// ---------------------------------------------------------------------------
+// W-stationary forward log-likelihood (Bernoulli, bf16 X, hi/lo W planes):
+// the sum over rows of -softplus(eta) for every chain, eta = W X^T.
+//
+// A workgroup (8 wave64s, one per CU) owns 64 chains and a contiguous range
+// of data rows.  The hi and lo planes of its 64 positions stay in LDS for
+// the whole launch (rows padded by 16 bytes: a ds_read_b128 pass over eight
+// or sixteen consecutive rows hits disjoint banks).  Each wave streams its
+// own 64 rows of X per 512-row tile straight from memory into MFMA operand
+// registers (lane l: row l & 31 of a 32-row block, bytes 16 (l >> 5) of a
+// 32-byte k step), kWsPre k steps ahead, with the prefetch running on into
+// the next tile while the previous tile's epilogue runs -- no LDS staging of
+// X and no block barrier in the main loop.  Per k step a wave issues 8 MFMAs
+// (2 planes x 2 chain blocks x 2 row blocks) accumulating each block over k
+// in the same order as gemm_nt_kernel, so eta is bit-identical to it.  The
+// epilogue of a tile adds -softplus over the wave's valid rows into FP64
+// per-lane sums; at the end lanes, then waves, are summed in a fixed order
+// into llpart[part][chain].
+// ---------------------------------------------------------------------------
+constexpr int kWsThreads = 512;
+constexpr int kWsBM = 64;                  // chains per workgroup
+constexpr int kWsWaveRows = 64;            // rows per wave per tile (2 blocks of 32)
+constexpr int kWsTile = 8 * kWsWaveRows;   // rows per tile
+constexpr int kWsPre = 4;                  // k steps of X in flight per wave
+constexpr int kWsMaxDPad = 512;
+constexpr int kWsRowBytes = 2 * kWsMaxDPad + 16;
+
+__global__ void __launch_bounds__(kWsThreads) __attribute__((amdgpu_waves_per_eu(1, 2)))
+gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
+                  const bf16_t* __restrict__ X, int64_t N, int64_t N_pad, int64_t rows_per_part,
+                  int parts, int mt_per_xcd, double* __restrict__ llpart, int64_t ll_ld) {
+  __shared__ __align__(16) char lds[2 * kWsBM * kWsRowBytes];
+  const int id = blockIdx.x, xcd = id & 7, local = id >> 3;
+  const int tm = xcd * mt_per_xcd + local % mt_per_xcd;
+  const int part = local / mt_per_xcd;
+  if (part >= parts) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int rsb = 2 * D_pad + 16;
+  // resident W: LDS row p * 64 + c = plane p of chain 64 tm + c
+  const int cpr = D_pad / 8;                 // 16-byte chunks per row
+  for (int q = tid; q < 2 * kWsBM * cpr; q += kWsThreads) {
+    const int row = q / cpr, ch = q - row * cpr;
+    const int pl = row / kWsBM, c = row - pl * kWsBM;
+    const uint4 v = *(const uint4*)(Wp + pl * plane_elems + (int64_t)(tm * kWsBM + c) * D_pad +
+                                    ch * 8);
+    *(uint4*)(lds + row * rsb + ch * 16) = v;
+  }
+  __syncthreads();
+
+  const int64_t n_begin = (int64_t)part * rows_per_part;
+  const int64_t n_end = min(n_begin + rows_per_part, N);
+  const int KS = D_pad / 16;                 // 32-byte k steps (a multiple of kWsPre)
+  const int64_t n_tiles = n_end > n_begin ? (n_end - n_begin + kWsTile - 1) / kWsTile : 0;
+  const int lrow = lane & 31, lhalf = (lane >> 5) * 16;
+  // byte address of this lane's X fragment row (block j of the wave's rows
+  // in tile t) at k step 0; rows past the padding are clamped (never used)
+  auto row_ptr = [&](int64_t t, int j) -> const char* {
+    int64_t n = n_begin + t * kWsTile + wave * kWsWaveRows + j * 32 + lrow;
+    n = n < N_pad ? n : N_pad - 1;
+    return (const char*)(X + n * D_pad) + lhalf;
+  };
+  double lls[2][16];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) lls[i][r] = 0.0;
+  if (n_tiles > 0) {
+    uint4 xr[kWsPre][2];
+    {
+      const char* rp0 = row_ptr(0, 0);
+      const char* rp1 = row_ptr(0, 1);
+#pragma unroll
+      for (int u = 0; u < kWsPre; ++u) {
+        xr[u][0] = *(const uint4*)(rp0 + u * 32);
+        xr[u][1] = *(const uint4*)(rp1 + u * 32);
+      }
+    }
+    f32x16_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+    // (t, ks0): the block of kWsPre k steps computed now; (tn, kn): the
+    // block whose loads are issued meanwhile (the last block again past the
+    // end: in bounds, never used)
+    int64_t t = 0;
+    int ks0 = 0;
+    while (t < n_tiles) {
+      int64_t tn = t;
+      int kn = ks0 + kWsPre;
+      if (kn == KS) { kn = 0; tn = t + 1; }
+      int64_t tl = tn;
+      int kl = kn;
+      if (tl >= n_tiles) { tl = n_tiles - 1; kl = KS - kWsPre; }
+      const char* rp0 = row_ptr(tl, 0) + kl * 32;
+      const char* rp1 = row_ptr(tl, 1) + kl * 32;
+#pragma unroll
+      for (int u = 0; u < kWsPre; ++u) {
+        const int ks = ks0 + u;
+        uint4 wf[2][2];
+#pragma unroll
+        for (int pl = 0; pl < 2; ++pl)
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+            wf[pl][i] = *(const uint4*)(lds + (pl * kWsBM + i * 32 + lrow) * rsb + ks * 32 + lhalf);
+#pragma unroll
+        for (int pl = 0; pl < 2; ++pl)
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) Mfma<bf16_t>::chunk(acc[i][j], wf[pl][i], xr[u][j]);
+        xr[u][0] = *(const uint4*)(rp0 + u * 32);
+        xr[u][1] = *(const uint4*)(rp1 + u * 32);
+      }
+      if (ks0 + kWsPre == KS) {
+        // end of tile t: -softplus over the wave's valid rows
+        const int64_t nb = n_begin + t * kWsTile + wave * kWsWaveRows + lrow;
+        const bool v0 = nb < n_end, v1 = nb + 32 < n_end;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            float s32 = 0.0f;
+            s32 -= v0 ? softplus_fast(acc[i][0][r]) : 0.0f;
+            s32 -= v1 ? softplus_fast(acc[i][1][r]) : 0.0f;
+            lls[i][r] += (double)s32;
+            acc[i][0][r] = 0.0f;
+            acc[i][1][r] = 0.0f;
+          }
+      }
+      t = tn;
+      ks0 = kn;
+    }
+  }
+  // lanes with equal lane >> 5 hold the same chains at different rows
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      double v = lls[i][r];
+#pragma unroll
+      for (int off = 1; off < 32; off <<= 1) v += __shfl_xor(v, off, 64);
+      lls[i][r] = v;
+    }
+  __syncthreads();   // the W tiles are no longer read: reuse LDS for the wave sums
+  double* red = reinterpret_cast<double*>(lds);   // [8 waves][64 chains]
+  if (lrow == 0) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) red[wave * kWsBM + 32 * i + acc_row(r, lane)] = lls[i][r];
+  }
+  __syncthreads();
+  if (tid < kWsBM) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWsThreads / 64; ++w) v += red[w * kWsBM + tid];
+    llpart[(int64_t)part * ll_ld + tm * kWsBM + tid] = v;
+  }
+}
+
+// Whether the W-stationary forward applies: Bernoulli, bf16, D_pad <= 512,
+// chain tiles a multiple of 8 (one slice per XCD), enough rows to split.
+static bool ws_forward_ok(const edhmc_ctx* ctx) {
+#ifdef EDHMC_NO_WS_FORWARD
+  return false;
+#endif
+  return ctx->cdt == EDHMC_BF16 && ctx->lik == EDHMC_LIK_BERNOULLI && ctx->D_pad <= kWsMaxDPad &&
+         (ctx->D_pad / 16) % kWsPre == 0 && ctx->C_pad % (8 * kWsBM) == 0 &&
+         ctx->N >= 16 * kWsTile;
+}
+
+static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st) {
+  const int m_tiles = ctx->C_pad / kWsBM;
+  // about one workgroup per CU: split the rows of each chain tile into parts
+  int parts = std::max(1, 256 / m_tiles);
+  int64_t rows_per_part = (ctx->N + parts - 1) / parts;
+  rows_per_part = (rows_per_part + kWsTile - 1) / kWsTile * kWsTile;
+  parts = (int)((ctx->N + rows_per_part - 1) / rows_per_part);
+  if (parts > ctx->n_tiles1) return set_err(ctx, EDHMC_EUNSUPPORTED, "forward: too many parts");
+  const int mt_per_xcd = m_tiles / 8;
+  const int grid = 8 * mt_per_xcd * parts;
+  ProfScope ps(ctx, PK_GEMM_LL, st);
+  hipLaunchKernelGGL(gemm_ll_ws_kernel, dim3(grid), dim3(kWsThreads), 0, st,
+                     (const bf16_t*)ctx->z_lp.p, (int64_t)ctx->C_pad * ctx->D_pad, ctx->D_pad,
+                     (const bf16_t*)ctx->Xr.p, ctx->N, ctx->N_pad, rows_per_part, parts,
+                     mt_per_xcd, (double*)ctx->llpart.p, (int64_t)ctx->C_pad);
+  EDHMC_TRY(check_launch(ctx, "gemm_ll_ws"));
+  return ll_reduce(ctx, parts, st);
+}
+
 // Forward only (log-likelihood partials, no residuals) at z_lp, exact form.
 int glm_forward_ll(edhmc_ctx* ctx, hipStream_t st) {
-  if (ctx->cdt == EDHMC_BF16) EDHMC_TRY((launch_gemm1<bf16_t, CfgP2, 2, false, true>(ctx, st)));
+  if (ws_forward_ok(ctx)) EDHMC_TRY(launch_ws_forward(ctx, st));
+  else if (ctx->cdt == EDHMC_BF16) EDHMC_TRY((launch_gemm1<bf16_t, CfgP2, 2, false, true>(ctx, st)));
   else EDHMC_TRY((launch_gemm1<float, Cfg1, 1, false, true>(ctx, st)));
   if (ctx->lik != EDHMC_LIK_BERNOULLI) return EDHMC_OK;
   hipLaunchKernelGGL(ll_linear_kernel, dim3(ctx->C), dim3(256), 0, st, (const void*)ctx->z_lp.p,

@@ -305,3 +305,21 @@ def test_device_ess_matches_oracle(between):
   dev = diagnostics.ess(eng, torch.as_tensor(xf, device='cuda'), between=between)
   ref = np.array([O.ess(xf[:, :, d].astype(np.float64).T, between=between) for d in range(D)])
   np.testing.assert_allclose(dev, ref, rtol=1e-6)
+
+
+@pytest.mark.parametrize('N,D,C', [(40000, 200, 512), (20000, 512, 1024)])
+def test_forward_only_logp_chain_stationary_path(N, D, C):
+  """Sizes that take the W-stationary forward kernel (bf16, Bernoulli,
+  chain tiles a multiple of 8, D_pad <= 512): log joint against the oracle
+  (rtol 1e-5, X rounded to bf16 on both sides) and against the log joint of
+  edhmc_logp_grad (its GEMM epilogue sums the same eta)."""
+  from edward_amd import workloads as W
+  print('case N=%d D=%d C=%d' % (N, D, C), flush=True)
+  wl = W.logreg(N, D, C, 0.01, 10, 10, seed=31, bf16=True)
+  eng = _engine(wl, precision='bf16')
+  W0 = (np.random.default_rng(32).normal(size=(C, D)) * 0.05).astype(np.float32)
+  lp = eng.logp(W0).cpu().numpy()
+  lp_ref, _ = _oracle(wl).logp_grad(W0.astype(np.float64))
+  np.testing.assert_allclose(lp, lp_ref, rtol=1e-5)
+  lpg, _ = eng.logp_grad(W0)
+  np.testing.assert_allclose(lp, lpg.cpu().numpy(), rtol=2e-6)
