@@ -93,10 +93,14 @@ __device__ inline float log1p_unit(float e) {
 // transcendentals: t = 2^(-|eta| log2 e) in (0, 1]; log1p(t) as ln2 log2(1 + t),
 // or t (1 - t/2) once 1 + t would round t away (t < 2^-12: the dropped t^3/3
 // is below 2^-36).  Absolute error about 2^-24 per element.
+// Both forms are evaluated and one is selected: a branch per element (what
+// the compiler makes of a conditional around the logarithm) costs an exec
+// mask save/restore pair in the middle of the epilogues.
 __device__ inline float softplus_fast(float eta) {
   const float t = __builtin_amdgcn_exp2f(-fabsf(eta) * 1.4426950408889634f);
-  const float l = t < 2.44140625e-4f ? t * (1.0f - 0.5f * t)
-                                     : __builtin_amdgcn_logf(1.0f + t) * 0.6931471805599453f;
+  const float big = __builtin_amdgcn_logf(1.0f + t) * 0.6931471805599453f;
+  const float small = t * (1.0f - 0.5f * t);
+  const float l = t < 2.44140625e-4f ? small : big;
   return fmaxf(eta, 0.0f) + l;
 }
 

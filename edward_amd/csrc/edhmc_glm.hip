@@ -171,7 +171,10 @@ struct EpiGLM {
         if (sp_only) {
           float s32 = 0.0f;
 #pragma unroll
-          for (int j = 0; j < WN; ++j) s32 -= valid[j] ? softplus_fast(acc[i][j][r]) : 0.0f;
+          for (int j = 0; j < WN; ++j) {
+            const float sp = softplus_fast(acc[i][j][r]);   // unconditional: no branch
+            s32 -= valid[j] ? sp : 0.0f;
+          }
           llsum = (double)s32;
         } else {
 #pragma unroll
@@ -1168,7 +1171,10 @@ constexpr int kWsThreads = 512;
 constexpr int kWsBM = 64;                  // chains per workgroup
 constexpr int kWsWaveRows = 64;            // rows per wave per tile (2 blocks of 32)
 constexpr int kWsTile = 8 * kWsWaveRows;   // rows per tile
-constexpr int kWsPre = 4;                  // k steps of X in flight per wave
+#ifndef EDHMC_WS_PRE
+#define EDHMC_WS_PRE 4
+#endif
+constexpr int kWsPre = EDHMC_WS_PRE;       // k steps of X in flight per wave
 constexpr int kWsMaxDPad = 512;
 constexpr int kWsRowBytes = 2 * kWsMaxDPad + 16;
 
@@ -1269,9 +1275,10 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
         for (int i = 0; i < 2; ++i)
 #pragma unroll
           for (int r = 0; r < 16; ++r) {
+            const float sp0 = softplus_fast(acc[i][0][r]), sp1 = softplus_fast(acc[i][1][r]);
             float s32 = 0.0f;
-            s32 -= v0 ? softplus_fast(acc[i][0][r]) : 0.0f;
-            s32 -= v1 ? softplus_fast(acc[i][1][r]) : 0.0f;
+            s32 -= v0 ? sp0 : 0.0f;
+            s32 -= v1 ? sp1 : 0.0f;
             lls[i][r] += (double)s32;
             acc[i][0][r] = 0.0f;
             acc[i][1][r] = 0.0f;
