@@ -1156,27 +1156,35 @@ int glm_ensure_chains(edhmc_ctx* ctx, int C);
 // of data rows.  The hi and lo planes of its 64 positions stay in LDS for
 // the whole launch (rows padded by 16 bytes: a ds_read_b128 pass over eight
 // or sixteen consecutive rows hits disjoint banks).  Each wave streams its
-// own 64 rows of X per 512-row tile straight from memory into MFMA operand
-// registers (lane l: row l & 31 of a 32-row block, bytes 16 (l >> 5) of a
-// 32-byte k step), kWsPre k steps ahead, with the prefetch running on into
-// the next tile while the previous tile's epilogue runs -- no LDS staging of
-// X and no block barrier in the main loop.  Per k step a wave issues 8 MFMAs
-// (2 planes x 2 chain blocks x 2 row blocks) accumulating each block over k
-// in the same order as gemm_nt_kernel, so eta is bit-identical to it.  The
-// epilogue of a tile adds -softplus over the wave's valid rows into FP64
-// per-lane sums; at the end lanes, then waves, are summed in a fixed order
-// into llpart[part][chain].
+// own 64 rows of X per 512-row tile straight from memory into the operand
+// registers of v_mfma_f32_16x16x32_bf16: for a 16-row block and a 64-byte
+// k step, lane l holds row l & 15, bytes 16 (l >> 4) .. +16, so the four
+// lanes of a row read 64 contiguous bytes (16 rows per load instruction,
+// half the cache lines per byte of the 32 x 32 layout).  kWsPre k steps of X
+// are in flight, the prefetch running on into the next tile while the
+// previous tile's epilogue runs; no LDS staging of X and no block barrier in
+// the main loop.  Per k step a wave issues 32 MFMAs (2 planes x 4 chain
+// blocks x 4 row blocks).  The epilogue of a tile adds -softplus over the
+// wave's valid rows into FP64 per-lane sums; at the end lanes, then waves,
+// are summed in a fixed order into llpart[part][chain].
 // ---------------------------------------------------------------------------
 constexpr int kWsThreads = 512;
 constexpr int kWsBM = 64;                  // chains per workgroup
-constexpr int kWsWaveRows = 64;            // rows per wave per tile (2 blocks of 32)
+constexpr int kWsWaveRows = 64;            // rows per wave per tile (4 blocks of 16)
 constexpr int kWsTile = 8 * kWsWaveRows;   // rows per tile
 #ifndef EDHMC_WS_PRE
 #define EDHMC_WS_PRE 4
 #endif
-constexpr int kWsPre = EDHMC_WS_PRE;       // k steps of X in flight per wave
+constexpr int kWsPre = EDHMC_WS_PRE;       // 64-byte k steps of X in flight per wave
 constexpr int kWsMaxDPad = 512;
 constexpr int kWsRowBytes = 2 * kWsMaxDPad + 16;
+
+typedef __attribute__((ext_vector_type(4))) float f32x4v_t;
+
+__device__ inline void mfma16(f32x4v_t& acc, const uint4& a, const uint4& b) {
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                __builtin_bit_cast(bf16x8_t, b), acc, 0, 0, 0);
+}
 
 __global__ void __launch_bounds__(kWsThreads) __attribute__((amdgpu_waves_per_eu(1, 2)))
 gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
@@ -1202,39 +1210,38 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
 
   const int64_t n_begin = (int64_t)part * rows_per_part;
   const int64_t n_end = min(n_begin + rows_per_part, N);
-  const int KS = D_pad / 16;                 // 32-byte k steps (a multiple of kWsPre)
+  const int KS = D_pad / 32;                 // 64-byte k steps (a multiple of kWsPre)
   const int64_t n_tiles = n_end > n_begin ? (n_end - n_begin + kWsTile - 1) / kWsTile : 0;
-  const int lrow = lane & 31, lhalf = (lane >> 5) * 16;
+  const int l16 = lane & 15, lq = (lane >> 4) * 16;
   // byte address of this lane's X fragment row (block j of the wave's rows
   // in tile t) at k step 0; rows past the padding are clamped (never used)
   auto row_ptr = [&](int64_t t, int j) -> const char* {
-    int64_t n = n_begin + t * kWsTile + wave * kWsWaveRows + j * 32 + lrow;
+    int64_t n = n_begin + t * kWsTile + wave * kWsWaveRows + j * 16 + l16;
     n = n < N_pad ? n : N_pad - 1;
-    return (const char*)(X + n * D_pad) + lhalf;
+    return (const char*)(X + n * D_pad) + lq;
   };
-  double lls[2][16];
+  // lls[i][v]: chain 16 i + 4 (lane >> 4) + v, summed over this lane's rows
+  double lls[4][4];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int r = 0; r < 16; ++r) lls[i][r] = 0.0;
+    for (int v = 0; v < 4; ++v) lls[i][v] = 0.0;
   if (n_tiles > 0) {
-    uint4 xr[kWsPre][2];
+    uint4 xr[kWsPre][4];
     {
-      const char* rp0 = row_ptr(0, 0);
-      const char* rp1 = row_ptr(0, 1);
+      const char* rp[4];
 #pragma unroll
-      for (int u = 0; u < kWsPre; ++u) {
-        xr[u][0] = *(const uint4*)(rp0 + u * 32);
-        xr[u][1] = *(const uint4*)(rp1 + u * 32);
-      }
+      for (int j = 0; j < 4; ++j) rp[j] = row_ptr(0, j);
+#pragma unroll
+      for (int u = 0; u < kWsPre; ++u)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) xr[u][j] = *(const uint4*)(rp[j] + u * 64);
     }
-    f32x16_t acc[2][2];
+    f32x4v_t acc[4][4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+      for (int j = 0; j < 4; ++j) acc[i][j] = f32x4v_t{0.0f, 0.0f, 0.0f, 0.0f};
     // (t, ks0): the block of kWsPre k steps computed now; (tn, kn): the
     // block whose loads are issued meanwhile (the last block again past the
     // end: in bounds, never used)
@@ -1247,71 +1254,77 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
       int64_t tl = tn;
       int kl = kn;
       if (tl >= n_tiles) { tl = n_tiles - 1; kl = KS - kWsPre; }
-      const char* rp0 = row_ptr(tl, 0) + kl * 32;
-      const char* rp1 = row_ptr(tl, 1) + kl * 32;
+      const char* rp[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) rp[j] = row_ptr(tl, j) + kl * 64;
 #pragma unroll
       for (int u = 0; u < kWsPre; ++u) {
         const int ks = ks0 + u;
-        uint4 wf[2][2];
 #pragma unroll
-        for (int pl = 0; pl < 2; ++pl)
+        for (int pl = 0; pl < 2; ++pl) {
+          uint4 wf[4];
 #pragma unroll
-          for (int i = 0; i < 2; ++i)
-            wf[pl][i] = *(const uint4*)(lds + (pl * kWsBM + i * 32 + lrow) * rsb + ks * 32 + lhalf);
+          for (int i = 0; i < 4; ++i)
+            wf[i] = *(const uint4*)(lds + (pl * kWsBM + i * 16 + l16) * rsb + ks * 64 + lq);
 #pragma unroll
-        for (int pl = 0; pl < 2; ++pl)
+          for (int i = 0; i < 4; ++i)
 #pragma unroll
-          for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 4; ++j) mfma16(acc[i][j], wf[i], xr[u][j]);
+        }
 #pragma unroll
-            for (int j = 0; j < 2; ++j) Mfma<bf16_t>::chunk(acc[i][j], wf[pl][i], xr[u][j]);
-        xr[u][0] = *(const uint4*)(rp0 + u * 32);
-        xr[u][1] = *(const uint4*)(rp1 + u * 32);
+        for (int j = 0; j < 4; ++j) xr[u][j] = *(const uint4*)(rp[j] + u * 64);
       }
       if (ks0 + kWsPre == KS) {
-        // end of tile t: -softplus over the wave's valid rows
-        const int64_t nb = n_begin + t * kWsTile + wave * kWsWaveRows + lrow;
-        const bool v0 = nb < n_end, v1 = nb + 32 < n_end;
+        // end of tile t: -softplus over the wave's valid rows (row j*16 + l16)
+        const int64_t nb = n_begin + t * kWsTile + wave * kWsWaveRows + l16;
+        bool valid[4];
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 4; ++j) valid[j] = nb + 16 * j < n_end;
 #pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            const float sp0 = softplus_fast(acc[i][0][r]), sp1 = softplus_fast(acc[i][1][r]);
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
             float s32 = 0.0f;
-            s32 -= v0 ? sp0 : 0.0f;
-            s32 -= v1 ? sp1 : 0.0f;
-            lls[i][r] += (double)s32;
-            acc[i][0][r] = 0.0f;
-            acc[i][1][r] = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const float sp = softplus_fast(acc[i][j][v]);   // unconditional: no branch
+              s32 -= valid[j] ? sp : 0.0f;
+            }
+            lls[i][v] += (double)s32;
           }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = f32x4v_t{0.0f, 0.0f, 0.0f, 0.0f};
       }
       t = tn;
       ks0 = kn;
     }
   }
-  // lanes with equal lane >> 5 hold the same chains at different rows
+  // lanes with equal lane >> 4 hold the same chains at different rows
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      double v = lls[i][r];
+    for (int v = 0; v < 4; ++v) {
+      double x = lls[i][v];
 #pragma unroll
-      for (int off = 1; off < 32; off <<= 1) v += __shfl_xor(v, off, 64);
-      lls[i][r] = v;
+      for (int off = 1; off < 16; off <<= 1) x += __shfl_xor(x, off, 64);
+      lls[i][v] = x;
     }
   __syncthreads();   // the W tiles are no longer read: reuse LDS for the wave sums
   double* red = reinterpret_cast<double*>(lds);   // [8 waves][64 chains]
-  if (lrow == 0) {
+  if (l16 == 0) {
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) red[wave * kWsBM + 32 * i + acc_row(r, lane)] = lls[i][r];
+      for (int v = 0; v < 4; ++v) red[wave * kWsBM + 16 * i + (lane >> 4) * 4 + v] = lls[i][v];
   }
   __syncthreads();
   if (tid < kWsBM) {
-    double v = 0.0;
+    double x = 0.0;
 #pragma unroll
-    for (int w = 0; w < kWsThreads / 64; ++w) v += red[w * kWsBM + tid];
-    llpart[(int64_t)part * ll_ld + tm * kWsBM + tid] = v;
+    for (int w = 0; w < kWsThreads / 64; ++w) x += red[w * kWsBM + tid];
+    llpart[(int64_t)part * ll_ld + tm * kWsBM + tid] = x;
   }
 }
 
@@ -1322,7 +1335,7 @@ static bool ws_forward_ok(const edhmc_ctx* ctx) {
   return false;
 #endif
   return ctx->cdt == EDHMC_BF16 && ctx->lik == EDHMC_LIK_BERNOULLI && ctx->D_pad <= kWsMaxDPad &&
-         (ctx->D_pad / 16) % kWsPre == 0 && ctx->C_pad % (8 * kWsBM) == 0 &&
+         (ctx->D_pad / 32) % kWsPre == 0 && ctx->C_pad % (8 * kWsBM) == 0 &&
          ctx->N >= 16 * kWsTile;
 }
 
