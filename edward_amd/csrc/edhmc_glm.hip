@@ -1189,11 +1189,22 @@ __device__ inline void mfma16(f32x4v_t& acc, const uint4& a, const uint4& b) {
 __global__ void __launch_bounds__(kWsThreads) __attribute__((amdgpu_waves_per_eu(1, 2)))
 gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
                   const bf16_t* __restrict__ X, int64_t N, int64_t N_pad, int64_t rows_per_part,
-                  int parts, int mt_per_xcd, double* __restrict__ llpart, int64_t ll_ld) {
+                  int parts, int mt_per_xcd, int part_per_xcd, double* __restrict__ llpart,
+                  int64_t ll_ld) {
   __shared__ __align__(16) char lds[2 * kWsBM * kWsRowBytes];
+  // Consecutive workgroup ids go to the 8 XCDs in turn.  With part_per_xcd
+  // (parts divides 8) every XCD streams one row range for 8 / parts slices
+  // of its chain tiles, so all its workgroups read the same X rows through
+  // its L2; otherwise an XCD owns mt_per_xcd chain tiles over all parts.
   const int id = blockIdx.x, xcd = id & 7, local = id >> 3;
-  const int tm = xcd * mt_per_xcd + local % mt_per_xcd;
-  const int part = local / mt_per_xcd;
+  int tm, part;
+  if (part_per_xcd) {
+    part = xcd % parts;
+    tm = (xcd / parts) * mt_per_xcd + local;
+  } else {
+    tm = xcd * mt_per_xcd + local % mt_per_xcd;
+    part = local / mt_per_xcd;
+  }
   if (part >= parts) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int rsb = 2 * D_pad + 16;
@@ -1347,13 +1358,20 @@ static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st) {
   rows_per_part = (rows_per_part + kWsTile - 1) / kWsTile * kWsTile;
   parts = (int)((ctx->N + rows_per_part - 1) / rows_per_part);
   if (parts > ctx->n_tiles1) return set_err(ctx, EDHMC_EUNSUPPORTED, "forward: too many parts");
-  const int mt_per_xcd = m_tiles / 8;
-  const int grid = 8 * mt_per_xcd * parts;
+#ifndef EDHMC_WS_PART_PER_XCD
+#define EDHMC_WS_PART_PER_XCD 1
+#endif
+  const bool part_per_xcd = EDHMC_WS_PART_PER_XCD && 8 % parts == 0;
+  // chain tiles per XCD: all of them split over the XCDs sharing a part, or
+  // an eighth of them over all parts
+  const int mt_per_xcd = part_per_xcd ? m_tiles / (8 / parts) : m_tiles / 8;
+  const int grid = m_tiles * parts;
   ProfScope ps(ctx, PK_GEMM_LL, st);
   hipLaunchKernelGGL(gemm_ll_ws_kernel, dim3(grid), dim3(kWsThreads), 0, st,
                      (const bf16_t*)ctx->z_lp.p, (int64_t)ctx->C_pad * ctx->D_pad, ctx->D_pad,
                      (const bf16_t*)ctx->Xr.p, ctx->N, ctx->N_pad, rows_per_part, parts,
-                     mt_per_xcd, (double*)ctx->llpart.p, (int64_t)ctx->C_pad);
+                     mt_per_xcd, part_per_xcd ? 1 : 0, (double*)ctx->llpart.p,
+                     (int64_t)ctx->C_pad);
   EDHMC_TRY(check_launch(ctx, "gemm_ll_ws"));
   return ll_reduce(ctx, parts, st);
 }

@@ -307,10 +307,12 @@ def test_device_ess_matches_oracle(between):
   np.testing.assert_allclose(dev, ref, rtol=1e-6)
 
 
-@pytest.mark.parametrize('N,D,C', [(40000, 200, 512), (20000, 512, 1024)])
+@pytest.mark.parametrize('N,D,C', [(40000, 200, 512), (20000, 512, 1024), (30000, 128, 2048),
+                                   (30000, 128, 4096)])
 def test_forward_only_logp_chain_stationary_path(N, D, C):
   """Sizes that take the W-stationary forward kernel (bf16, Bernoulli,
-  chain tiles a multiple of 8, D_pad <= 512): log joint against the oracle
+  chain tiles a multiple of 8, D_pad <= 512; the last two split the rows
+  into 8 and 4 parts, one per XCD or pair of XCDs): log joint against the oracle
   (rtol 1e-5, X rounded to bf16 on both sides) and against the log joint of
   edhmc_logp_grad (its GEMM epilogue sums the same eta)."""
   from edward_amd import workloads as W
