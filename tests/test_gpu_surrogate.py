@@ -47,7 +47,7 @@ def _rel(a, b):
   return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
 
 
-@pytest.mark.parametrize('D,H,B', [(24, 64, 300), (40, 200, 129)])
+@pytest.mark.parametrize('D,H,B', [(24, 64, 300), (40, 200, 129), (24, 1000, 300)])
 def test_loss_grad_and_forward_match_oracle(D, H, B):
   _, eng = _engine(D=D)
   sur = _surrogate(eng, H=H, B=512, seed=1)
