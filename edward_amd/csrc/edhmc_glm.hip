@@ -1177,7 +1177,11 @@ constexpr int kWsTile = 8 * kWsWaveRows;   // rows per tile
 #endif
 constexpr int kWsPre = EDHMC_WS_PRE;       // 64-byte k steps of X in flight per wave
 constexpr int kWsMaxDPad = 512;
-constexpr int kWsRowBytes = 2 * kWsMaxDPad + 16;
+#ifndef EDHMC_WS_PAD
+#define EDHMC_WS_PAD 16
+#endif
+constexpr int kWsPad = EDHMC_WS_PAD;       // bytes of padding per resident W row
+constexpr int kWsRowBytes = 2 * kWsMaxDPad + kWsPad;
 
 typedef __attribute__((ext_vector_type(4))) float f32x4v_t;
 
@@ -1207,7 +1211,7 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
   }
   if (part >= parts) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int rsb = 2 * D_pad + 16;
+  const int rsb = 2 * D_pad + kWsPad;
   // resident W: LDS row p * 64 + c = plane p of chain 64 tm + c
   const int cpr = D_pad / 8;                 // 16-byte chunks per row
   for (int q = tid; q < 2 * kWsBM * cpr; q += kWsThreads) {
