@@ -80,3 +80,23 @@ def test_laplace_draws_and_gradient_pairs():
   Wp, Gp = lap.gradient_pairs(1000, 256, seed=2)
   _, g_ref = m.logp_grad(Wp.cpu().numpy().astype(np.float64))
   assert np.abs(Gp.cpu().numpy() - g_ref).max() <= 1e-4 * np.abs(g_ref).max()
+
+
+def test_newton_terms_reject_what_they_do_not_model():
+  """Constrained prior families and a likelihood weight other than 1 are not
+  modelled by the Newton terms: the call raises instead of returning a wrong
+  Hessian."""
+  from edward_amd import _lib as L
+  eng, m, w_true = _case(5, 2000, 8)
+  eng.set_lik_weight(0.5)
+  with pytest.raises(L.EdhmcError) as err:
+    eng.newton_terms(w_true.astype(np.float32))
+  assert err.value.code == L.EUNSUPPORTED
+  eng.set_lik_weight(1.0)
+  eng.set_prior_family(np.full(8, 16, np.uint8), np.full(8, -0.9189385332046727))
+  with pytest.raises(L.EdhmcError) as err:
+    eng.newton_terms(w_true.astype(np.float32))
+  assert err.value.code == L.EUNSUPPORTED
+  eng.set_prior_family(None)
+  lp, g, P = eng.newton_terms(w_true.astype(np.float32))
+  assert np.isfinite(lp) and np.all(np.isfinite(P))

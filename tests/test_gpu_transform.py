@@ -245,3 +245,31 @@ def test_gamma_weight_logistic_regression_matches_oracle_hmc():
     assert abs(w_dev[..., j].mean() - d[..., j].mean()) < 4 * np.hypot(se(w_dev[..., j]),
                                                                     se(d[..., j]))
   assert np.all(a_dev > 0)
+
+
+def test_fed_data_rebinds_with_the_prior_families():
+  """A placeholder fed through update(feed_dict=...) rebinds the data; the
+  constrained prior families are applied again (the log joint of the rebound
+  engine equals the oracle's with the Gamma term)."""
+  rng = np.random.default_rng(9)
+  N = 300
+  Xd = rng.normal(size=(N, 2)).astype(np.float32)
+  yd = (rng.random(N) < 0.5).astype(np.int32)
+  w = Normal(loc=np.zeros(1, np.float32), scale=np.ones(1, np.float32))
+  A = Gamma(np.full(1, 2.0, np.float32), np.ones(1, np.float32))
+  Xw = ed.placeholder('float32', [N, 1])
+  Xa = ed.placeholder('float32', [N, 1])
+  y = Bernoulli(logits=ed.dot(Xw, w) + ed.dot(Xa, A))
+  qw = Empirical(np.zeros((5, 8, 1), np.float32))
+  qa = Empirical(np.zeros((5, 8, 1), np.float32))
+  inf = ed.HMC({w: qw, A: qa}, data={y: yd})
+  inf.initialize(step_size=0.05, n_steps=3, auto_transform=True, seed=1)
+  inf.update(feed_dict={Xw: Xd[:, :1], Xa: Xd[:, 1:]})
+  inf.update(feed_dict={Xw: Xd[:, 1:], Xa: Xd[:, :1]})     # second feed: rebind
+  U = np.array([[0.3, -0.5], [-1.0, 2.0]], np.float32)
+  lp, _ = inf.engine.logp_grad(U)
+  m = TO.TransformedGLM(np.concatenate([Xd[:, 1:], Xd[:, :1]], 1).astype(np.float64), yd,
+                        [TO.FAMILY_NORMAL, TO.FAMILY_GAMMA], [TO.OP_IDENTITY, TO.OP_SOFTPLUS],
+                        [0.0, 2.0], [1.0, 1.0])
+  lp_ref, _ = m.logp_grad(U.astype(np.float64))
+  np.testing.assert_allclose(lp.cpu().numpy(), lp_ref, rtol=1e-6, atol=1e-3)

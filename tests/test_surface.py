@@ -145,3 +145,16 @@ def test_empirical_moments_on_cpu_match_the_reference_definition():
   np.testing.assert_allclose(q.mean().cpu().numpy(), p.mean(0), rtol=1e-6)
   np.testing.assert_allclose(q.stddev().cpu().numpy(), p.std(0), rtol=1e-5)
   np.testing.assert_allclose(q.mean(burn_in=3).cpu().numpy(), p[3:].mean(0), rtol=1e-6)
+
+
+def test_replica_exchange_checks_the_temperatures():
+  """``replica_exchange_mc.py:59-63``: the first inverse temperature must be 1."""
+  import edward_amd as ed
+  from edward_amd.models import Empirical, Normal
+  z = Normal(0.0, 1.0)
+  with pytest.raises(ValueError):
+    ed.ReplicaExchangeMC({z: Empirical(np.zeros(10, np.float32))},
+                         {z: Normal(loc=z, scale=0.5)}, inverse_temperatures=[0.5, 0.25])
+  inf = ed.ReplicaExchangeMC({z: Empirical(np.zeros(10, np.float32))},
+                             {z: Normal(loc=z, scale=0.5)}, inverse_temperatures=[1.0, 0.3])
+  assert inf.n_replica == 2 and len(inf.replica_vars) == 2
