@@ -51,6 +51,8 @@ SIGNATURES = {
     'edhmc_set_lik_weight': (ctypes.c_int, [_c_p, _f32]),
     'edhmc_set_prior_family': (ctypes.c_int, [_c_p, _c_p, _c_p, _c_p]),
     'edhmc_newton_terms': (ctypes.c_int, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
+    'edhmc_hier_set_scale_prior': (ctypes.c_int, [_c_p, _i32, ctypes.c_double, ctypes.c_double,
+                                                   ctypes.c_double]),
     'edhmc_mh_step': (ctypes.c_int, [_c_p, _c_p, _c_p, _u64, _u64, _c_p, _c_p, _c_p, _c_p]),
     'edhmc_replica_exchange_step': (ctypes.c_int, [_c_p, _c_p, _c_p, _i32, _f32, _u64, _u64,
                                                    _c_p, _c_p, _c_p, _c_p, _c_p]),

@@ -367,11 +367,8 @@ def glm_spec(latent_vars, data, scale=None, transforms=None):
             _has_gather(obs.logits):
       if scale:
         raise NotImplementedError('scale is not supported for the hierarchical model')
-      if transforms and any(v is not None for v in transforms.values()):
-        raise NotImplementedError('constrained latents are not supported by the '
-                                  'hierarchical model')
       from edward_amd import hier
-      return hier.compile_hier(latent_vars, data, obs, yv)
+      return hier.compile_hier(latent_vars, data, obs, yv, transforms)
     if isinstance(obs, M.Bernoulli):
       if obs.logits is None:
         if not (isinstance(obs.probs, M.RandomVariable) and obs.probs in latent_vars and

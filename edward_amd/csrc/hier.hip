@@ -24,6 +24,7 @@
 #include "common.h"
 #include "ctx.h"
 #include "philox.h"
+#include "transform.h"
 
 namespace edhmc {
 
@@ -33,6 +34,11 @@ int glm_ensure_chains(edhmc_ctx* ctx, int C);
 
 struct HierModel {
   int G = 0, D = 0, dim = 0, dim_pad = 0;
+  // prior of the group-scale coordinate u (see HierArgs::skind); default:
+  // u = log sigma ~ Normal(0, 1)
+  int skind = kPriorNormal;
+  float sp1 = 0.0f, sp2 = 1.0f;
+  double slnorm = -0.5 * kLog2Pi;
   int64_t N = 0;
   DevBuf gidx, offsets, ones, zeros;
   int C = 0, C_pad = 0;
@@ -97,7 +103,23 @@ struct HierArgs {
   const float* r_fed; const double* u_fed; uint64_t seed, counter;
   float* sample_out; int32_t* accepted_out; int32_t* n_accept; int32_t* ndiv;
   const float* W_in; float* grad_out; double* logp_out;
+  // group-scale coordinate u: sigma = exp(u) (skind >> 4 == 0) or
+  // softplus(u) (== 1); its prior is the family skind & 15 of transform.h
+  // in u-space with parameters sp1, sp2 and constant slnorm
+  int skind; float sp1, sp2; double slnorm;
 };
+
+// sigma and d log sigma / du of the group-scale coordinate
+__device__ inline void hier_sigma(const HierArgs& a, float u, float& sig, double& dlog) {
+  if ((a.skind >> 4) == 0) {
+    sig = expf(u);
+    dlog = 1.0;
+  } else {
+    const double sp = softplus_f64((double)u);
+    sig = (float)sp;
+    dlog = sig_over_softplus_f64((double)u);
+  }
+}
 
 __device__ inline double nlogpdf(double x, double m, double s) {
   const double t = (x - m) / s;
@@ -109,8 +131,11 @@ __device__ inline void hier_assemble(const HierArgs& a, int c, bool want_lp, dou
                                      double& logp) {
   const float* th = a.th + (int64_t)c * a.dim_pad;
   const float mu = th[a.G], ls = th[a.G + 1];
-  const float sig = expf(ls);
+  float sig;
+  double dlog;
+  hier_sigma(a, ls, sig, dlog);
   const float inv2 = 1.0f / (sig * sig);
+  const PriorTerm sprior = prior_term(a.skind & 15, (double)ls, a.sp1, a.sp2, a.slnorm);
   double s1 = 0.0, s2 = 0.0, plp = 0.0;
   for (int j = threadIdx.x; j < a.G; j += kHT) {
     const double z = ((double)th[j] - mu) / sig;
@@ -128,7 +153,8 @@ __device__ inline void hier_assemble(const HierArgs& a, int c, bool want_lp, dou
     } else if (j == a.G) {
       gv = (float)(s1 - (double)mu);
     } else if (j == a.G + 1) {
-      gv = (float)((s2 - (double)a.G) - (double)ls);
+      // d/du sum_g log N(a_g; mu, sigma) = (s2 - G) d log sigma / du
+      gv = (float)((s2 - (double)a.G) * dlog + sprior.dlp);
     } else {
       const int d = j - a.G - 2;
       const float* gp = a.Gpart + (int64_t)c * a.D_pad + d;
@@ -139,7 +165,7 @@ __device__ inline void hier_assemble(const HierArgs& a, int c, bool want_lp, dou
     g[j] = gv;
     if (want_lp) {
       if (j < a.G) plp += nlogpdf(th[j], mu, sig);
-      else if (j == a.G || j == a.G + 1) plp += nlogpdf(th[j], 0.0, 1.0);
+      else if (j == a.G + 1) plp += sprior.lp;
       else plp += nlogpdf(th[j], 0.0, 1.0);
     }
   }
@@ -294,6 +320,7 @@ static HierArgs hier_args(edhmc_ctx* ctx, HierModel* h) {
   a.logp = (double*)h->logp.p; a.K0 = (double*)h->K0.p;
   a.zlp = (float*)ctx->z_lp.p;
   a.ndiv = (int32_t*)ctx->ndiv.p;   // divergence counts share the GLM slot
+  a.skind = h->skind; a.sp1 = h->sp1; a.sp2 = h->sp2; a.slnorm = h->slnorm;
   return a;
 }
 
@@ -431,6 +458,26 @@ int edhmc_bind_hier(edhmc_ctx* ctx, const float* X, const float* y, const int64_
                      (const int64_t*)h->offsets.p, G, (int*)h->gidx.p);
   EDHMC_TRY(check_launch(ctx, "group_index"));
   if (hipStreamSynchronize(st) != hipSuccess) return set_err(ctx, EDHMC_EHIP, "sync");
+  return EDHMC_OK;
+}
+
+int edhmc_hier_set_scale_prior(edhmc_ctx* ctx, int32_t kind, double p1, double p2,
+                               double lnorm) {
+  if (!ctx) return EDHMC_EINVAL;
+  HierModel* h = ctx->hier;
+  if (!h) return set_err(ctx, EDHMC_ESTATE, "no hierarchical model bound");
+  const int fam = kind & 15, map = kind >> 4;
+  if (kind < 0 || map > 1 || fam > kPriorInvGamma || fam == kPriorBeta)
+    return set_err(ctx, EDHMC_EINVAL, "scale prior kind %d: family Normal, Gamma or "
+                   "InverseGamma, map exp (0) or softplus (1)", kind);
+  if (!(p2 > 0.0) || !std::isfinite(p1) || !std::isfinite(p2) || !std::isfinite(lnorm) ||
+      (fam != kPriorNormal && !(p1 > 0.0)))
+    return set_err(ctx, EDHMC_EINVAL, "scale prior parameters must be finite and positive");
+  h->skind = kind;
+  h->sp1 = (float)p1;
+  h->sp2 = (float)p2;
+  h->slnorm = lnorm;
+  h->state_valid = false;   // the carried log joint and gradient used the old prior
   return EDHMC_OK;
 }
 

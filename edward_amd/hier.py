@@ -1,11 +1,15 @@
 """Hierarchical logistic regression engine (batched over chains).
 
-Model (latent vector per chain ``[a (G), mu, log_sigma, w (D)]``)::
+Model (latent vector per chain ``[a (G), mu, u, w (D)]``)::
 
-  mu ~ Normal(0, 1);  log_sigma ~ Normal(0, 1)
+  mu ~ Normal(0, 1);  log_sigma ~ Normal(m, s)   (u = log_sigma)
   a  ~ Normal(mu, exp(log_sigma))          # one intercept per group, shape [G]
   w  ~ Normal(0, 1)                        # shape [D]
   y  ~ Bernoulli(logits=ed.gather(a, group) + ed.dot(X, w))
+
+or, with a positive group scale auto-transformed (u = softplus^-1(sigma),
+``inference.py:222-264``), ``a ~ Normal(mu, sigma)`` with ``sigma`` ~ Gamma,
+Exponential, Chi2, InverseGamma, or a Normal pushed through Softplus.
 
 This is the gather model of ``tests/inferences/hmc_test.py:105-123`` grown to
 a multilevel regression; its log joint is ``hmc.py:182-192``'s sum and the
@@ -49,6 +53,12 @@ class HierEngine(Context):
     self.check(self.lib.edhmc_bind_hier(self.h, L.ptr(Xd), L.ptr(yd), L.ptr(off), self.G, N, D,
                                         self.stream))
     torch.cuda.synchronize(dev)
+
+  def set_scale_prior(self, kind, p1, p2, lnorm):
+    """Prior of the group-scale coordinate (``edhmc_hier_set_scale_prior``):
+    kind = family | map << 4 (map 0: sigma = exp(u), 1: sigma = softplus(u))."""
+    self.check(self.lib.edhmc_hier_set_scale_prior(self.h, int(kind), float(p1), float(p2),
+                                                   float(lnorm)))
 
   def _theta(self, theta):
     t = _dev_tensor(theta, torch.float32, self.device)
@@ -109,10 +119,38 @@ def _is_std_normal(rv):
   return isinstance(rv, M.Normal) and const(rv.loc, 0.0) and const(rv.scale, 1.0)
 
 
-def compile_hier(latent_vars, data, obs, yv):
+def _scale_prior(sc, latent_vars, transforms):
+  """(latent of the group-scale coordinate, kind, p1, p2, lnorm) of the
+  scale argument of the intercepts' Normal."""
+  from edward_amd import compile as Cm
+  if isinstance(sc, M.Exp) and isinstance(sc.x, M.Normal):
+    ls = sc.x
+    if transforms.get(ls) is not None:
+      raise NotImplementedError('log_sigma must have real support')
+    loc, scale = Cm._resolve(ls.loc, {}), Cm._resolve(ls.scale, {})
+    if loc is None or scale is None or np.size(loc) != 1 or np.size(scale) != 1:
+      raise NotImplementedError('log_sigma needs a scalar Normal prior with constant '
+                                'parameters')
+    fam = Cm.PriorFamily(Cm.FAMILY_NORMAL, np.ravel(loc), np.ravel(scale), 0, 1)
+    return ls, int(fam.family[0]), float(fam.p1[0]), float(fam.p2[0]), float(fam.lnorm[0])
+  if isinstance(sc, M.RandomVariable) and sc in latent_vars:
+    if transforms.get(sc) != 'softplus':
+      raise NotImplementedError('a positive group scale must be auto-transformed '
+                                '(initialize with auto_transform=True)')
+    if sc.shape:
+      raise NotImplementedError('the group scale must be a scalar')
+    fam = Cm._constrained_family(sc, 1, 'softplus')
+    return (sc, int(fam.family[0]) | (1 << 4), float(fam.p1[0]), float(fam.p2[0]),
+            float(fam.lnorm[0]))
+  raise NotImplementedError('a must be Normal(mu, exp(log_sigma)) with log_sigma ~ Normal, '
+                            'or Normal(mu, sigma) with a positive latent sigma')
+
+
+def compile_hier(latent_vars, data, obs, yv, transforms=None):
   """Match ``Bernoulli(logits=gather(a, group) + dot(X, w))`` and build the
   engine; raises NotImplementedError for any other gather model."""
   from edward_amd.compile import Latent, Program, _array, _chains_of, _resolve
+  transforms = transforms or {}
   E = obs.logits
   parts = [E.a, E.b]
   ga = [p for p in parts if isinstance(p, M.Gather)]
@@ -132,18 +170,18 @@ def compile_hier(latent_vars, data, obs, yv):
   if X is None or X.ndim != 2 or not _is_std_normal(w):
     raise NotImplementedError('dot(X, w) needs a data matrix and w ~ Normal(0, 1)')
   mu = a.loc
-  sc = a.scale
-  if not (isinstance(mu, M.Normal) and _is_std_normal(mu) and isinstance(sc, M.Exp) and
-          isinstance(sc.x, M.Normal) and _is_std_normal(sc.x)):
-    raise NotImplementedError('a must be Normal(mu, exp(log_sigma)) with mu, log_sigma ~ '
-                              'Normal(0, 1)')
-  ls = sc.x
+  if not (isinstance(mu, M.Normal) and _is_std_normal(mu)):
+    raise NotImplementedError('the intercepts\' mean must be mu ~ Normal(0, 1)')
+  for rv in (a, mu, w):
+    if transforms.get(rv) is not None:
+      raise NotImplementedError('%s must have real support' % rv.name)
+  ls, skind, sp1, sp2, slnorm = _scale_prior(a.scale, latent_vars, transforms)
   for rv in (a, mu, ls, w):
     if rv not in latent_vars:
       raise NotImplementedError('%s must be a latent variable of the inference' % rv.name)
   if len(latent_vars) != 4:
     raise NotImplementedError('the hierarchical model has exactly the latents a, mu, '
-                              'log_sigma, w')
+                              'the group scale, w')
   G = int(np.prod(a.shape))
   D = int(np.prod(w.shape))
   if X.shape[1] != D:
@@ -161,5 +199,6 @@ def compile_hier(latent_vars, data, obs, yv):
     lat.append(Latent(rv, qz, k, off, C))
     off += k
   eng = HierEngine(Xs.astype(np.float32), ys.astype(np.float32), offsets)
+  eng.set_scale_prior(skind, sp1, sp2, slnorm)
   T = int(min(l.qz.params.shape[0] for l in lat))
   return Program(eng, lat, T, n_chains, off)

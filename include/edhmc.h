@@ -346,6 +346,17 @@ int edhmc_bind_hier(edhmc_ctx* ctx, const float* X, const float* y, const int64_
                     int32_t G, int64_t N, int32_t D, void* stream);
 int edhmc_hier_logp_grad(edhmc_ctx* ctx, const float* theta, int32_t C, float* grad_out,
                          double* logp_out, void* stream);
+/* Prior of the group-scale coordinate u of the hierarchical model (bind
+ * sets u = log sigma ~ Normal(0, 1)).  kind = family | map << 4: family 0
+ * Normal(p1, p2) on u, 1 Gamma(p1, p2) or 3 InverseGamma(p1, p2) on sigma
+ * with the log-Jacobian of the map (edward/inferences/inference.py:222-264,
+ * hmc.py:132-159); map 0: sigma = exp(u) (a log-scale latent), 1: sigma =
+ * softplus(u) (a positive latent auto-transformed by inverse softplus,
+ * util/random_variables.py:856-917).  lnorm is the family's normalising
+ * constant.  Invalidates the chain state (set it again). */
+int edhmc_hier_set_scale_prior(edhmc_ctx* ctx, int32_t kind, double p1, double p2,
+                               double lnorm);
+
 int edhmc_hier_set_state(edhmc_ctx* ctx, const float* theta, int32_t C, void* stream);
 int edhmc_hier_transition(edhmc_ctx* ctx, float eps, int32_t n_steps, const float* r0,
                           const double* u, uint64_t seed, uint64_t counter,

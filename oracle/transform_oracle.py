@@ -21,7 +21,8 @@ import math
 
 import numpy as np
 
-from oracle.hmc_oracle import GLM, sigmoid, softplus
+from oracle.hmc_oracle import (GLM, HierLogReg, bernoulli_logits_logp, normal_logp, sigmoid,
+                               softplus)
 
 FAMILY_NORMAL, FAMILY_GAMMA, FAMILY_BETA, FAMILY_INVGAMMA = 0, 1, 2, 3
 OP_IDENTITY, OP_SOFTPLUS, OP_SIGMOID = 0, 1, 2
@@ -138,6 +139,47 @@ class TransformedGLM(object):
 
   def logp(self, U):
     return self.logp_grad(U)[0]
+
+
+class ScaledHierLogReg(HierLogReg):
+  """Hierarchical logistic regression with a general group-scale prior.
+
+  theta = [a (G), mu, u, w (D)] with sigma = exp(u) (``scale_map='exp'``, u
+  is log_sigma ~ Normal(p1, p2)) or sigma = softplus(u) (``'softplus'``: a
+  positive latent sigma auto-transformed as in ``inference.py:222-264``, its
+  prior family pushed to u-space with the log-Jacobian, ``hmc.py:132-159``).
+  The other terms are ``HierLogReg``'s.
+  """
+
+  def __init__(self, X, y, offsets, scale_map='exp', family=FAMILY_NORMAL, p1=0.0, p2=1.0):
+    super(ScaledHierLogReg, self).__init__(X, y, offsets)
+    self.scale_map, self.family, self.p1, self.p2 = scale_map, family, p1, p2
+
+  def sigma(self, u):
+    return np.exp(u) if self.scale_map == 'exp' else softplus(u)
+
+  def logp_grad(self, theta):
+    a, mu, u, w = self.split(theta)
+    sig = self.sigma(u)
+    dsig = sig if self.scale_map == 'exp' else sigmoid(u)
+    eta = a[:, self.group] + w @ self.X.T
+    ll = bernoulli_logits_logp(self.y, eta).sum(axis=1)
+    res = self.y - sigmoid(eta)
+    za = (a - mu[:, None]) / sig[:, None]
+    su, dsu = prior_logp_grad(u, self.family, self.p1, self.p2)
+    lp = (normal_logp(a, mu[:, None], sig[:, None]).sum(axis=1)
+          + normal_logp(mu, 0.0, 1.0) + su + normal_logp(w, 0.0, 1.0).sum(axis=1))
+    ga = np.zeros_like(a)
+    for g in range(self.G):
+      lo, hi = self.offsets[g], self.offsets[g + 1]
+      ga[:, g] = res[:, lo:hi].sum(axis=1)
+    ga += -za / sig[:, None]
+    gmu = (za / sig[:, None]).sum(axis=1) - mu
+    # d/dsigma sum_g log N(a_g; mu, sigma) = sum_g (z_g^2 - 1) / sigma
+    gu = ((za ** 2 - 1.0).sum(axis=1) / sig) * dsig + dsu
+    gw = res @ self.X - w
+    grad = np.concatenate([ga, gmu[:, None], gu[:, None], gw], axis=1)
+    return ll + lp, grad
 
 
 # ----------------------------------------------------------------------------

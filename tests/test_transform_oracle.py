@@ -298,3 +298,95 @@ def test_constrained_latents_need_hmc_and_support():
   inf._auto_transform(True)
   with pytest.raises(NotImplementedError):
     inf._compile_transforms()
+
+
+def _hier_data(seed=3, sizes=(5, 0, 9, 3), D=3):
+  rng = np.random.default_rng(seed)
+  off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+  N = int(off[-1])
+  X = rng.standard_normal((N, D))
+  y = (rng.random(N) < 0.5).astype(np.float64)
+  return X, y, off
+
+
+def test_scaled_hier_reduces_to_the_log_sigma_model():
+  X, y, off = _hier_data()
+  base = O.HierLogReg(X, y, off)
+  m = TO.ScaledHierLogReg(X, y, off, 'exp', TO.FAMILY_NORMAL, 0.0, 1.0)
+  th = np.random.default_rng(1).standard_normal((4, base.dim))
+  lp0, g0 = base.logp_grad(th)
+  lp1, g1 = m.logp_grad(th)
+  np.testing.assert_allclose(lp1, lp0, rtol=1e-13)
+  np.testing.assert_allclose(g1, g0, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize('smap,family,a,b', [
+    ('exp', TO.FAMILY_NORMAL, 0.3, 0.7),
+    ('softplus', TO.FAMILY_GAMMA, 2.0, 3.0),
+    ('softplus', TO.FAMILY_INVGAMMA, 3.0, 1.5),
+    ('softplus', TO.FAMILY_NORMAL, -0.4, 0.6),
+])
+def test_scaled_hier_scipy_log_joint_and_gradient(smap, family, a, b):
+  """log p against scipy.stats in the constrained space plus the log-Jacobian
+  of sigma(u), and the gradient against central differences."""
+  X, y, off = _hier_data()
+  m = TO.ScaledHierLogReg(X, y, off, smap, family, a, b)
+  G = m.G
+  rng = np.random.default_rng(7)
+  th = 0.6 * rng.standard_normal((3, m.dim))
+  lp, g = m.logp_grad(th)
+  for c in range(th.shape[0]):
+    av, mu, u, w = th[c, :G], th[c, G], th[c, G + 1], th[c, G + 2:]
+    sig = math.exp(u) if smap == 'exp' else math.log1p(math.exp(u))
+    eta = av[m.group] + X @ w
+    ref = st.bernoulli.logpmf(y, 1.0 / (1.0 + np.exp(-eta))).sum()
+    ref += st.norm.logpdf(av, mu, sig).sum() + st.norm.logpdf(mu) + st.norm.logpdf(w).sum()
+    if family == TO.FAMILY_NORMAL:
+      ref += st.norm.logpdf(u, a, b)          # the density of u itself
+    elif family == TO.FAMILY_GAMMA:
+      ref += st.gamma.logpdf(sig, a, scale=1.0 / b) + math.log(1.0 / (1.0 + math.exp(-u)))
+    else:
+      ref += st.invgamma.logpdf(sig, a, scale=b) + math.log(1.0 / (1.0 + math.exp(-u)))
+    np.testing.assert_allclose(lp[c], ref, rtol=1e-10)
+  h = 1e-6
+  for j in range(m.dim):
+    e = np.zeros(m.dim)
+    e[j] = h
+    fd = (m.logp_grad(th + e)[0] - m.logp_grad(th - e)[0]) / (2 * h)
+    np.testing.assert_allclose(g[:, j], fd, rtol=1e-6, atol=1e-6)
+
+
+def test_hier_scale_prior_kinds():
+  """The group-scale prior the hierarchical compiler hands the device:
+  kind = family | map << 4 with map 0 for exp(log_sigma), 1 for a softplus-
+  transformed positive latent, and the host-side normalising constant."""
+  from edward_amd import hier as H
+  ls = Normal(loc=0.5, scale=2.0)
+  kind, p1, p2, lnorm = H._scale_prior(ed.exp(ls), {ls: None}, {})[1:]
+  assert (kind, p1, p2) == (TO.FAMILY_NORMAL, 0.5, 2.0)
+  np.testing.assert_allclose(lnorm, -math.log(2.0) - 0.5 * math.log(2 * math.pi), rtol=1e-6)
+  for rv, fam, a, b in [(Gamma(concentration=2.0, rate=3.0), TO.FAMILY_GAMMA, 2.0, 3.0),
+                        (Exponential(rate=1.5), TO.FAMILY_GAMMA, 1.0, 1.5),
+                        (Chi2(df=3.0), TO.FAMILY_GAMMA, 1.5, 0.5),
+                        (InverseGamma(concentration=3.0, rate=1.5), TO.FAMILY_INVGAMMA, 3.0,
+                         1.5)]:
+    sc, kind, p1, p2, lnorm = H._scale_prior(rv, {rv: None}, {rv: 'softplus'})
+    assert sc is rv and kind == fam | (1 << 4) and (p1, p2) == (a, b)
+    # lnorm + the u-density's other terms integrate to one over u
+    # (the InverseGamma's sigma^-4 tail needs the long second range)
+    mass = 0.0
+    for u in (np.linspace(-40.0, 40.0, 400001), np.linspace(40.0, 4000.0, 400001)):
+      mass += np.trapezoid(np.exp(TO.prior_logp_grad(u, fam, a, b)[0]), u)
+    np.testing.assert_allclose(mass, 1.0, rtol=1e-6)
+    np.testing.assert_allclose(lnorm, a * math.log(b) - math.lgamma(a), rtol=1e-6)
+  sp = TransformedDistribution(distribution=Normal(loc=-0.4, scale=0.6), bijector=B.Softplus())
+  assert H._scale_prior(sp, {sp: None}, {sp: 'softplus'})[1:3] == (TO.FAMILY_NORMAL | 16,
+                                                                   np.float32(-0.4))
+  g = Gamma(concentration=2.0, rate=3.0)
+  with pytest.raises(NotImplementedError):
+    H._scale_prior(g, {g: None}, {})                # not auto-transformed
+  bt = Beta(concentration1=2.0, concentration0=2.0)
+  with pytest.raises(NotImplementedError):
+    H._scale_prior(bt, {bt: None}, {bt: 'sigmoid'})
+  with pytest.raises(NotImplementedError):
+    H._scale_prior(ed.exp(Gamma(concentration=2.0, rate=1.0)), {}, {})
