@@ -170,8 +170,11 @@ struct EpiPos {
   int smem_bytes; // its size
 };
 
+#ifndef EDHMC_GEMM_WPE
+#define EDHMC_GEMM_WPE 2
+#endif
 template <typename T, typename Cfg, int A_PLANES, typename Epi>
-__global__ void __launch_bounds__(Cfg::kThreads) __attribute__((amdgpu_waves_per_eu(1, 2)))
+__global__ void __launch_bounds__(Cfg::kThreads) __attribute__((amdgpu_waves_per_eu(1, EDHMC_GEMM_WPE)))
 gemm_nt_kernel(GemmShape g, Epi epi) {
   constexpr int NT = Cfg::kThreads;
   constexpr int WM = Cfg::kWM, WN = Cfg::kWN, WAVES_M = Cfg::kWavesM;

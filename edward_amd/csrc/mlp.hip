@@ -21,13 +21,20 @@
 
 #include "common.h"
 #include "ctx.h"
+// the layer GEMMs' small tiles need few registers: allow four waves per SIMD
+#define EDHMC_GEMM_WPE 4
 #include "gemm_nt.h"
 #include "philox.h"
 
 namespace edhmc {
 
 using CfgM = TileCfg<2, 2, 2, 2>;    // 128 x 128 tiles
-using CfgMS = TileCfg<1, 1, 2, 2>;   // 64 x 64 tiles for GEMMs with few output tiles
+// 64 x 64 tiles with 128-byte K stages for GEMMs with few output tiles (the
+// 4096-chain layers: 1024 workgroups of 4 waves, four resident per CU)
+using CfgMS = TileCfg<1, 1, 2, 2, 128>;
+#ifndef EDHMC_MLP_THRESH
+#define EDHMC_MLP_THRESH 512
+#endif
 
 constexpr float kGeluK0 = 0.7978845608028654f;  // sqrt(2 / pi)
 constexpr float kGeluK1 = 0.044715f;
@@ -361,17 +368,19 @@ static int mlp_gemm_cfg(edhmc_ctx* ctx, const void* A, int64_t lda_elems, const 
   GemmShape g = {};
   g.A = (const char*)A;
   g.lda_b = lda_elems * 2;
-  g.a_stage_b = kStageBytes;
+  g.a_stage_b = Cfg::KS;
   g.a_plane_b = 0;
   g.B = (const char*)B;
   g.ldb_b = ldb_elems * 2;
-  g.b_stage_b = kStageBytes;
+  g.b_stage_b = Cfg::KS;
   g.m_tiles = M / Cfg::BM;
   g.n_tiles = N / Cfg::BN;
-  g.k_stages = K * 2 / kStageBytes;
+  g.k_stages = K * 2 / Cfg::KS;
   g.ksplit = 1;
   g.stages_per_split = g.k_stages;
   g.swz = 0;
+  if (M % Cfg::BM || N % Cfg::BN || (K * 2) % Cfg::KS)
+    return set_err(ctx, EDHMC_EUNSUPPORTED, "MLP GEMM %d x %d x %d does not fit the tiles", M, N, K);
   if (!gemm_offsets_fit<Cfg, 1>(g)) return set_err(ctx, EDHMC_EUNSUPPORTED, "MLP tile too large");
   ProfScope ps(ctx, PK_MLP, st);
   hipLaunchKernelGGL((gemm_nt_kernel<bf16_t, Cfg, 1, Epi>), dim3(grid_size(g)),
@@ -384,7 +393,7 @@ static int mlp_gemm_cfg(edhmc_ctx* ctx, const void* A, int64_t lda_elems, const 
 template <typename Epi>
 static int mlp_gemm(edhmc_ctx* ctx, const void* A, int64_t lda_elems, const void* B,
                     int64_t ldb_elems, int M, int N, int K, Epi e, hipStream_t st) {
-  if ((M / CfgM::BM) * (N / CfgM::BN) < 512)
+  if ((M / CfgM::BM) * (N / CfgM::BN) < EDHMC_MLP_THRESH)
     return mlp_gemm_cfg<CfgMS>(ctx, A, lda_elems, B, ldb_elems, M, N, K, e, st);
   return mlp_gemm_cfg<CfgM>(ctx, A, lda_elems, B, ldb_elems, M, N, K, e, st);
 }
