@@ -71,6 +71,7 @@ SIGNATURES = {
     'edhmc_profile_enable': (ctypes.c_int, [_c_p, _i32]),
     'edhmc_profile_read': (ctypes.c_int, [_c_p, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(_i64)]),
+    'edhmc_profile_select': (ctypes.c_int, [_c_p, ctypes.c_uint32]),
     'edhmc_draw_momenta': (ctypes.c_int, [_c_p, _i32, _i32, _u64, _u64, _c_p, _c_p]),
     'edhmc_draw_uniforms': (ctypes.c_int, [_c_p, _i32, _u64, _u64, _c_p, _c_p]),
     'edhmc_autocov': (ctypes.c_int, [_c_p, _c_p, _i64, _i32, _i32, _i32, _c_p, _c_p, _c_p]),

@@ -32,6 +32,7 @@ enum ProfKind {
 // collection time.  Off unless enabled.
 struct Profiler {
   bool on = false;
+  uint32_t mask = ~0u;   // kernel classes (bit k = class k) that get events
   std::vector<hipEvent_t> pool;
   std::vector<std::pair<int, int>> marks;   // (kind, index of start event)
   double total_ms[PK_COUNT] = {};

@@ -23,7 +23,7 @@
 namespace edhmc {
 
 ProfScope::ProfScope(edhmc_ctx* c, int k, hipStream_t s) : ctx(c), st(s), kind(k) {
-  if (!ctx->prof.on) return;
+  if (!ctx->prof.on || !((ctx->prof.mask >> k) & 1u)) return;
   if (ctx->prof.marks.size() >= 4096) prof_collect(ctx);
   hipEvent_t e = ctx->prof.get();
   if (!e) return;
@@ -2095,6 +2095,12 @@ int edhmc_profile_enable(edhmc_ctx* ctx, int32_t enable) {
     ctx->prof.total_ms[k] = 0.0;
     ctx->prof.count[k] = 0;
   }
+  return EDHMC_OK;
+}
+
+int edhmc_profile_select(edhmc_ctx* ctx, uint32_t class_mask) {
+  if (!ctx) return EDHMC_EINVAL;
+  ctx->prof.mask = class_mask;
   return EDHMC_OK;
 }
 

@@ -67,8 +67,17 @@ class Context(object):
 
   PROFILE_KINDS = ('gemm_grad', 'gemm_ll', 'gemm_xtr', 'chain', 'mlp', 'hier')
 
-  def profile(self, on=True):
-    """Turn the library's per-launch hipEvent timing on (clears totals) or off."""
+  def profile(self, on=True, kinds=None):
+    """Turn the library's per-launch hipEvent timing on (clears totals) or off.
+    ``kinds``: the classes of ``PROFILE_KINDS`` to time (default all); each
+    timed launch adds two event records to its stream."""
+    if kinds is None:
+      mask = 0xffffffff
+    else:
+      mask = 0
+      for k in kinds:
+        mask |= 1 << self.PROFILE_KINDS.index(k)
+    self.check(self.lib.edhmc_profile_select(self.h, mask))
     self.check(self.lib.edhmc_profile_enable(self.h, 1 if on else 0))
 
   def profile_read(self):

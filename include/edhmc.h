@@ -244,6 +244,11 @@ int edhmc_moments_get(edhmc_ctx* ctx, double* mean_out, double* m2_out,
  * total_ms_out, count_out: host arrays of 6.  Enabling clears the totals. */
 int edhmc_profile_enable(edhmc_ctx* ctx, int32_t enable);
 int edhmc_profile_read(edhmc_ctx* ctx, double* total_ms_out, int64_t* count_out);
+/* Time only the classes whose bit is set (bit k = class k above; default all
+ * six).  Each timed launch costs two event records on its stream, about
+ * 10 us between dependent launches: time only the class being measured
+ * inside a timed region.  Classes left out report zero launches. */
+int edhmc_profile_select(edhmc_ctx* ctx, uint32_t class_mask);
 
 /* Record the (position, exact gradient) pair of every `stride`-th inner
  * leapfrog point (steps k = stride, 2 stride, ... < n_steps) of the following
