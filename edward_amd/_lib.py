@@ -84,6 +84,9 @@ SIGNATURES = {
     'edhmc_sur_loss_grad': (ctypes.c_int, [_c_p, _c_p, _c_p, _i32, _c_p, _c_p, _c_p]),
     'edhmc_sur_train_step': (ctypes.c_int, [_c_p, _c_p, _c_p, _i32, _f32, _f32, _f32, _f32,
                                             _i32, _c_p, _c_p]),
+    'edhmc_sur_train_epoch': (ctypes.c_int, [_c_p, _c_p, _c_p, _i64, _c_p, _i32, _i32,
+                                             ctypes.POINTER(_f32), _f32, _f32, _f32, _i32, _c_p,
+                                             _c_p]),
     'edhmc_transition_surrogate': (ctypes.c_int, [_c_p, _f32, _i32, _c_p, _c_p, _u64, _u64,
                                                   _c_p, _c_p, _c_p, _c_p]),
     'edhmc_bind_hier': (ctypes.c_int, [_c_p, _c_p, _c_p, _c_p, _i32, _i64, _i32, _c_p]),

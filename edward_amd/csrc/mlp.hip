@@ -68,6 +68,7 @@ struct Surrogate {
   DevBuf x0, a1, h1, a2, h2, o;        // activations (batch Bp); h2 is [Bp x Kc] = [h2 | x]
   DevBuf dO, da2, da1, x0t, h1t, h2t, dOt, da2t, da1t;
   DevBuf loss;                         // fp64 [1]
+  DevBuf Wb, Gb;                       // a gathered training batch [Bmax x D] fp32 each
   // sampling state
   DevBuf ghat, ghat0;                  // [C x D_pad of the GLM]
   bool ghat_valid = false;
@@ -83,7 +84,7 @@ void surrogate_destroy(Surrogate* s) {
   DevBuf* b[] = {&s->P, &s->Gr, &s->Am, &s->Av, &s->W1b, &s->W2b, &s->W3b, &s->W2t, &s->W3t,
                  &s->nrm, &s->x0, &s->a1, &s->h1, &s->a2, &s->h2, &s->o, &s->dO, &s->da2,
                  &s->da1, &s->x0t, &s->h1t, &s->h2t, &s->dOt, &s->da2t, &s->da1t, &s->loss,
-                 &s->ghat, &s->ghat0};
+                 &s->Wb, &s->Gb, &s->ghat, &s->ghat0};
   for (DevBuf* x : b) x->release();
   delete s;
 }
@@ -270,6 +271,17 @@ __global__ void norm_in_kernel(const float* __restrict__ W, int B, int D, int64_
   const bf16_t xb = f32_to_bf16(v);
   x0[i] = xb;
   xcat[(int64_t)b * xcat_ld + d] = xb;
+}
+
+// dst [B x D] = src rows idx[0..B) ([P x D]); a row whose index lies outside
+// [0, P) is written as NaN, so that the loss of its batch is NaN.
+__global__ void gather_rows_kernel(const float* __restrict__ src, int64_t P, int D,
+                                   const int64_t* __restrict__ idx, int B, float* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)B * D) return;
+  const int b = (int)(i / D), d = (int)(i - (int64_t)b * D);
+  const int64_t r = idx[b];
+  dst[i] = (r >= 0 && r < P) ? src[r * D + d] : __builtin_nanf("");
 }
 
 // out [C x R] = in [R x C]^T for bf16, 64 x 64 tiles (R, C multiples of 64).
@@ -943,6 +955,40 @@ int edhmc_sur_train_step(edhmc_ctx* ctx, const float* W, const float* G, int32_t
                      beta2, adam_eps, c1, c2);
   EDHMC_TRY(check_launch(ctx, "adam"));
   return refresh_weights(ctx, s, st);
+}
+
+int edhmc_sur_train_epoch(edhmc_ctx* ctx, const float* W, const float* G, int64_t P,
+                          const int64_t* perm, int32_t batch, int32_t n_batches,
+                          const float* lr_host, float beta1, float beta2, float adam_eps,
+                          int32_t step0, double* losses_out, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  EDHMC_TRY(check_sur(ctx));
+  Surrogate* s = ctx->sur;
+  EDHMC_TRY(batch_ok(ctx, s, batch));
+  if (!W || !G || !perm || !lr_host || !losses_out)
+    return set_err(ctx, EDHMC_EINVAL, "NULL argument");
+  if (n_batches < 0 || (int64_t)n_batches * batch > P)
+    return set_err(ctx, EDHMC_EINVAL, "%d batches of %d exceed the %lld pairs", n_batches, batch,
+                   (long long)P);
+  if (step0 < 0) return set_err(ctx, EDHMC_EINVAL, "Adam step count must be >= 0");
+  const size_t bytes = (size_t)s->Bmax * s->D * 4;
+  if (s->Wb.bytes != bytes) EDHMC_TRY(dev_alloc(ctx, s->Wb, bytes, false));
+  if (s->Gb.bytes != bytes) EDHMC_TRY(dev_alloc(ctx, s->Gb, bytes, false));
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned gb = blocks_for((int64_t)batch * s->D);
+  for (int b = 0; b < n_batches; ++b) {
+    const int64_t* idx = perm + (int64_t)b * batch;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(gb), dim3(256), 0, st, W, P, s->D, idx, batch,
+                       (float*)s->Wb.p);
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(gb), dim3(256), 0, st, G, P, s->D, idx, batch,
+                       (float*)s->Gb.p);
+    EDHMC_TRY(check_launch(ctx, "gather_rows"));
+    EDHMC_TRY(edhmc_sur_train_step(ctx, (const float*)s->Wb.p, (const float*)s->Gb.p, batch,
+                                   lr_host[b], beta1, beta2, adam_eps, step0 + b + 1,
+                                   losses_out + b, stream));
+  }
+  return EDHMC_OK;
 }
 
 int edhmc_transition_surrogate(edhmc_ctx* ctx, float eps, int32_t n_steps, const float* r0,

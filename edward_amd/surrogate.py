@@ -68,16 +68,17 @@ class Surrogate(object):
 
   def init_skip_least_squares(self, W, G, max_rows=32768, ridge=1e-6):
     """Start the output layer's linear skip at the least-squares map from the
-    normalised inputs to the normalised targets (on a row sample, FP64)."""
+    normalised inputs to the normalised targets (on a row sample: the Gram
+    products on the device in fp32, the D x D solve on the host in FP64)."""
     P = int(W.shape[0])
     idx = np.linspace(0, P - 1, min(P, max_rows)).astype(np.int64)
     sel = torch.as_tensor(idx, device=self.engine.device)
-    w = W.index_select(0, sel).double().cpu().numpy()
-    g = G.index_select(0, sel).double().cpu().numpy()
-    ins, insc, outs, outsc = [v.double().cpu().numpy() for v in self._norm]
-    x = (w - ins) * insc
-    t = (g - outs) / outsc
-    A = np.linalg.solve(x.T @ x + ridge * len(x) * np.eye(self.D), x.T @ t)   # t ~ x A
+    ins, insc, outs, outsc = [v.to(self.engine.device, torch.float32) for v in self._norm]
+    x = (W.index_select(0, sel) - ins) * insc
+    t = (G.index_select(0, sel) - outs) / outsc
+    xtx = (x.t() @ x).double().cpu().numpy()
+    xtt = (x.t() @ t).double().cpu().numpy()
+    A = np.linalg.solve(xtx + ridge * len(idx) * np.eye(self.D), xtt)   # t ~ x A
     flat = self.get_params().cpu().numpy()
     o = self._offsets()
     W3 = flat[o['W3']:o['b3']].reshape(self.D, self.H + self.D)
@@ -145,6 +146,23 @@ class Surrogate(object):
         float(beta2), float(eps), self.step_count, L.ptr(loss), self.engine.stream))
     return loss
 
+  def train_epoch(self, W, G, perm, batch, lrs, beta1=0.9, beta2=0.999, eps=1e-8):
+    """len(lrs) Adam steps, step b on the pairs (W, G) at the indices
+    perm[b*batch:(b+1)*batch] with learning rate lrs[b], in one library call
+    (no host round trip per step).  Returns the per-step losses (device
+    fp64)."""
+    dev = self.engine.device
+    n_batches = len(lrs)
+    lr_host = (ctypes.c_float * max(n_batches, 1))(*[float(x) for x in lrs])
+    losses = torch.empty(n_batches, dtype=torch.float64, device=dev)
+    perm = perm.to(dev, torch.int64).contiguous()
+    self.engine.check(self.engine.lib.edhmc_sur_train_epoch(
+        self.engine.h, L.ptr(W), L.ptr(G), int(W.shape[0]), L.ptr(perm), int(batch),
+        n_batches, lr_host, float(beta1), float(beta2), float(eps), self.step_count,
+        L.ptr(losses), self.engine.stream))
+    self.step_count += int(n_batches)
+    return losses
+
   def validation_loss(self, W, G):
     """Mean loss over (W, G) in batches of max_batch (no parameter update)."""
     tot, n = 0.0, 0
@@ -189,14 +207,9 @@ class Surrogate(object):
     step = 0
     for ep in range(epochs):
       perm = torch.randperm(P, generator=gen, device=self.engine.device)
-      acc = []
-      for b in range(n_batches):
-        idx = perm[b * batch:(b + 1) * batch]
-        Wb = W.index_select(0, idx).contiguous()
-        Gb = G.index_select(0, idx).contiguous()
-        acc.append(self.train_step(Wb, Gb, lr=lr * decay ** step))
-        step += 1
-      ep_loss = torch.cat(acc).mean().item()
+      lrs = [lr * decay ** (step + b) for b in range(n_batches)]
+      ep_loss = self.train_epoch(W, G, perm, batch, lrs).mean().item()
+      step += n_batches
       losses.append(ep_loss)
       msg = 'surrogate epoch %d loss %.5f' % (ep, ep_loss)
       if best is not None:

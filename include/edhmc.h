@@ -327,6 +327,17 @@ int edhmc_sur_loss_grad(edhmc_ctx* ctx, const float* W, const float* G, int32_t 
 int edhmc_sur_train_step(edhmc_ctx* ctx, const float* W, const float* G, int32_t B,
                          float lr, float beta1, float beta2, float adam_eps, int32_t step,
                          double* loss_out, void* stream);
+/* n_batches Adam steps in one call: batch b is the pairs W[perm[b*batch+i]],
+ * G[perm[b*batch+i]] (i < batch; W, G [P, D], perm int64 device), trained
+ * as edhmc_sur_train_step with learning rate lr_host[b] (host array) and
+ * Adam step step0 + b + 1; losses_out [n_batches] fp64 (device).
+ * Permutation entries outside [0, P) gather NaN rows (the batch's loss is
+ * NaN).  Equal to the sequence of single steps, without a host round trip
+ * per batch. */
+int edhmc_sur_train_epoch(edhmc_ctx* ctx, const float* W, const float* G, int64_t P,
+                          const int64_t* perm, int32_t batch, int32_t n_batches,
+                          const float* lr_host, float beta1, float beta2, float adam_eps,
+                          int32_t step0, double* losses_out, void* stream);
 /* HMC transition whose leapfrog forces come from the surrogate; the ratio
  * uses the exact log joint at the end point (bf16 X with a hi/lo bf16 split
  * of w, FP64 reduction), so the chain targets the exact posterior.  Same

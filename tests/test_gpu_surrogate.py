@@ -124,6 +124,47 @@ def test_adam_training_matches_oracle_step_by_step():
   assert _rel(sur.get_params().cpu().numpy(), flat) < 2e-2
 
 
+def test_train_epoch_equals_single_steps():
+  """One edhmc_sur_train_epoch call (batches gathered by a permutation on the
+  device, per-step learning rates) leaves the same weights, bit for bit, as
+  the same steps taken one by one on index_select batches, and the same
+  losses up to the last bit (the loss adds per-wave FP64 partials atomically,
+  in whatever order the waves finish); the oracle pins the single steps (the
+  test above)."""
+  D, H, B, nb = 32, 128, 256, 5
+  _, eng = _engine(D=D)
+  rng, mu, cov, field = _field_problem(D, seed=9)
+  Wtr = rng.multivariate_normal(mu, cov, size=B * nb + 37)
+  Gtr = field(Wtr)
+  norm = {'in_shift': Wtr.mean(0), 'in_scale': 1 / Wtr.std(0), 'out_shift': Gtr.mean(0),
+          'out_scale': Gtr.std(0)}
+  Wt = torch.tensor(Wtr, dtype=torch.float32, device='cuda')
+  Gt = torch.tensor(Gtr, dtype=torch.float32, device='cuda')
+  perm = torch.tensor(rng.permutation(Wtr.shape[0]), dtype=torch.int64, device='cuda')
+  lrs = [3e-3 * 0.9 ** b for b in range(nb)]
+  a = _surrogate(eng, H=H, B=B, seed=11)
+  _set_norm(a, norm)
+  la = a.train_epoch(Wt, Gt, perm, B, lrs).cpu().numpy()
+  pa = a.get_params().cpu().numpy()
+  b = _surrogate(eng, H=H, B=B, seed=11)
+  _set_norm(b, norm)
+  lb = []
+  for k in range(nb):
+    idx = perm[k * B:(k + 1) * B]
+    lb.append(b.train_step(Wt.index_select(0, idx).contiguous(),
+                           Gt.index_select(0, idx).contiguous(), lr=lrs[k]).item())
+  np.testing.assert_allclose(la, np.array(lb), rtol=1e-12)
+  np.testing.assert_array_equal(pa, b.get_params().cpu().numpy())
+  assert a.step_count == b.step_count == nb
+  # an index outside the pairs gathers a NaN row: that step's loss is NaN
+  bad = perm.clone()
+  bad[3] = Wtr.shape[0]
+  c = _surrogate(eng, H=H, B=B, seed=11)
+  _set_norm(c, norm)
+  lc = c.train_epoch(Wt, Gt, bad, B, lrs[:1]).cpu().numpy()
+  assert np.isnan(lc[0])
+
+
 def test_hidden_layers_learn_beyond_the_linear_skip():
   """Field -P (w - mu) + tanh(2 (w - mu)): the least-squares skip captures the
   linear part; training must cut the remaining error by 40% (the FP64
