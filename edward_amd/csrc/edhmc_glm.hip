@@ -1119,11 +1119,38 @@ static int launch_gemm2(edhmc_ctx* ctx, hipStream_t st) {
 
 // One gradient evaluation at z_lp.  split: use both bf16 planes; ll: write the
 // log-likelihood partials.
+static bool ws_forward_ok(const edhmc_ctx* ctx);
+static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st, bool wr);
+static bool ws_grad_ok(const edhmc_ctx* ctx) {
+#ifdef EDHMC_NO_WS_GRAD
+  return false;
+#endif
+  return ws_forward_ok(ctx);
+}
+
+// Bernoulli: add w . X^T y (FP64, per chain) to the reduced log-likelihood of
+// a pass whose epilogue summed only -softplus(eta).
+static int ll_linear(edhmc_ctx* ctx, hipStream_t st) {
+  if (ctx->lik != EDHMC_LIK_BERNOULLI) return EDHMC_OK;
+  hipLaunchKernelGGL(ll_linear_kernel, dim3(ctx->C), dim3(256), 0, st, (const void*)ctx->z_lp.p,
+                     ctx->cdt == EDHMC_BF16 ? 1 : 0, ctx->D, ctx->D_pad, ctx->C_pad,
+                     (const double*)ctx->xty.p, (double*)ctx->ll.p);
+  return check_launch(ctx, "ll_linear");
+}
+
 static int grad_eval(edhmc_ctx* ctx, bool split, bool ll, hipStream_t st) {
   if (ctx->cdt == EDHMC_BF16) {
     if (split) {
-      if (ll) EDHMC_TRY((launch_gemm1<bf16_t, CfgP2, 2, true, true>(ctx, st)));
-      else EDHMC_TRY((launch_gemm1<bf16_t, CfgP2, 2, true, false>(ctx, st)));
+      if (ll && ws_grad_ok(ctx)) {
+        // the W-stationary kernel writes the residuals beside its -softplus
+        // partials; the same log-likelihood formula as the forward-only pass
+        EDHMC_TRY(launch_ws_forward(ctx, st, true));
+        EDHMC_TRY(ll_linear(ctx, st));
+      } else if (ll) {
+        EDHMC_TRY((launch_gemm1<bf16_t, CfgP2, 2, true, true>(ctx, st)));
+      } else {
+        EDHMC_TRY((launch_gemm1<bf16_t, CfgP2, 2, true, false>(ctx, st)));
+      }
     } else {
       if (ll) EDHMC_TRY((launch_gemm1<bf16_t, Cfg1, 1, true, true>(ctx, st)));
       else EDHMC_TRY((launch_gemm1<bf16_t, Cfg1, 1, true, false>(ctx, st)));
@@ -1176,6 +1203,10 @@ constexpr int kWsTile = 8 * kWsWaveRows;   // rows per tile
 #define EDHMC_WS_PRE 4
 #endif
 constexpr int kWsPre = EDHMC_WS_PRE;       // 64-byte k steps of X in flight per wave
+#ifndef EDHMC_WS_PRE_GRAD
+#define EDHMC_WS_PRE_GRAD 2
+#endif
+constexpr int kWsPreGrad = EDHMC_WS_PRE_GRAD;   // the same with the residual epilogue
 constexpr int kWsMaxDPad = 512;
 #ifndef EDHMC_WS_PAD
 #define EDHMC_WS_PAD 16
@@ -1190,11 +1221,18 @@ __device__ inline void mfma16(f32x4v_t& acc, const uint4& a, const uint4& b) {
                                                 __builtin_bit_cast(bf16x8_t, b), acc, 0, 0, 0);
 }
 
+// WR: also the gradient pass's residuals r = y - sigmoid(eta) (bf16), written
+// to Rt in the K-blocked layout [N_pad / 32][C_pad][32] that GEMM2 reads
+// (0 for the padded rows N .. N_pad), beside the log-likelihood partials.
+// PRE: 64-byte k steps of X in flight per wave (fewer with the residual
+// epilogue, whose registers they would otherwise crowd out).
+template <bool WR, int PRE>
 __global__ void __launch_bounds__(kWsThreads) __attribute__((amdgpu_waves_per_eu(1, 2)))
 gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
                   const bf16_t* __restrict__ X, int64_t N, int64_t N_pad, int64_t rows_per_part,
                   int parts, int mt_per_xcd, int part_per_xcd, double* __restrict__ llpart,
-                  int64_t ll_ld) {
+                  int64_t ll_ld, const float* __restrict__ y, bf16_t* __restrict__ Rt,
+                  int64_t ldr) {
   __shared__ __align__(16) char lds[2 * kWsBM * kWsRowBytes];
   // Consecutive workgroup ids go to the 8 XCDs in turn.  With part_per_xcd
   // (parts divides 8) every XCD streams one row range for 8 / parts slices
@@ -1225,7 +1263,7 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
 
   const int64_t n_begin = (int64_t)part * rows_per_part;
   const int64_t n_end = min(n_begin + rows_per_part, N);
-  const int KS = D_pad / 32;                 // 64-byte k steps (a multiple of kWsPre)
+  const int KS = D_pad / 32;                 // 64-byte k steps (a multiple of PRE)
   const int64_t n_tiles = n_end > n_begin ? (n_end - n_begin + kWsTile - 1) / kWsTile : 0;
   const int l16 = lane & 15, lq = (lane >> 4) * 16;
   // byte address of this lane's X fragment row (block j of the wave's rows
@@ -1235,20 +1273,27 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
     n = n < N_pad ? n : N_pad - 1;
     return (const char*)(X + n * D_pad) + lq;
   };
-  // lls[i][v]: chain 16 i + 4 (lane >> 4) + v, summed over this lane's rows
+  // Forward (!WR): W is the MFMA's A operand and X its B operand, so
+  // acc[i][j][v] is chain 16 i + 4 (lane >> 4) + v at data row 16 j + l16 of
+  // the wave's rows; lls[i][v] sums that chain over this lane's rows.
+  // Gradient pass (WR): the operands are swapped, acc[i][j][v] is chain
+  // 16 i + l16 at row 16 j + 4 (lane >> 4) + v -- a lane's four values of
+  // one chain are consecutive rows, stored as one 8-byte piece -- and
+  // lls[i][0] sums chain 16 i + l16.  (The swapped layout costs the forward
+  // 6 %; the residual stores gain more.)
   double lls[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int v = 0; v < 4; ++v) lls[i][v] = 0.0;
   if (n_tiles > 0) {
-    uint4 xr[kWsPre][4];
+    uint4 xr[PRE][4];
     {
       const char* rp[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) rp[j] = row_ptr(0, j);
 #pragma unroll
-      for (int u = 0; u < kWsPre; ++u)
+      for (int u = 0; u < PRE; ++u)
 #pragma unroll
         for (int j = 0; j < 4; ++j) xr[u][j] = *(const uint4*)(rp[j] + u * 64);
     }
@@ -1257,23 +1302,23 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = f32x4v_t{0.0f, 0.0f, 0.0f, 0.0f};
-    // (t, ks0): the block of kWsPre k steps computed now; (tn, kn): the
+    // (t, ks0): the block of PRE k steps computed now; (tn, kn): the
     // block whose loads are issued meanwhile (the last block again past the
     // end: in bounds, never used)
     int64_t t = 0;
     int ks0 = 0;
     while (t < n_tiles) {
       int64_t tn = t;
-      int kn = ks0 + kWsPre;
+      int kn = ks0 + PRE;
       if (kn == KS) { kn = 0; tn = t + 1; }
       int64_t tl = tn;
       int kl = kn;
-      if (tl >= n_tiles) { tl = n_tiles - 1; kl = KS - kWsPre; }
+      if (tl >= n_tiles) { tl = n_tiles - 1; kl = KS - PRE; }
       const char* rp[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) rp[j] = row_ptr(tl, j) + kl * 64;
 #pragma unroll
-      for (int u = 0; u < kWsPre; ++u) {
+      for (int u = 0; u < PRE; ++u) {
         const int ks = ks0 + u;
 #pragma unroll
         for (int pl = 0; pl < 2; ++pl) {
@@ -1284,29 +1329,79 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
 #pragma unroll
           for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) mfma16(acc[i][j], wf[i], xr[u][j]);
+            for (int j = 0; j < 4; ++j) {
+              if (WR) mfma16(acc[i][j], xr[u][j], wf[i]);
+              else mfma16(acc[i][j], wf[i], xr[u][j]);
+            }
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) xr[u][j] = *(const uint4*)(rp[j] + u * 64);
       }
-      if (ks0 + kWsPre == KS) {
-        // end of tile t: -softplus over the wave's valid rows (row j*16 + l16)
-        const int64_t nb = n_begin + t * kWsTile + wave * kWsWaveRows + l16;
-        bool valid[4];
+      if (ks0 + PRE == KS) {
+        if (!WR) {
+          // end of tile t: -softplus over the wave's valid rows (row j*16 + l16)
+          const int64_t nb = n_begin + t * kWsTile + wave * kWsWaveRows + l16;
+          bool valid[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) valid[j] = nb + 16 * j < n_end;
+          for (int j = 0; j < 4; ++j) valid[j] = nb + 16 * j < n_end;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+          for (int i = 0; i < 4; ++i)
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            float s32 = 0.0f;
+            for (int v = 0; v < 4; ++v) {
+              float s32 = 0.0f;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              const float sp = softplus_fast(acc[i][j][v]);   // unconditional: no branch
-              s32 -= valid[j] ? sp : 0.0f;
+              for (int j = 0; j < 4; ++j) {
+                const float sp = softplus_fast(acc[i][j][v]);   // unconditional: no branch
+                s32 -= valid[j] ? sp : 0.0f;
+              }
+              lls[i][v] += (double)s32;
             }
-            lls[i][v] += (double)s32;
+        } else {
+          // end of tile t: this lane's rows are rb + 16 j + v
+          // residuals: a lane's 4 rows of one chain are 8 contiguous bytes of
+          // the chain's 64-byte K block; 32-bit offsets from the tile's first
+          // K block (its 512 rows span 16 blocks: below 16 * C_pad * 32
+          // elements).  Rt holds rows up to N_pad rounded up to a whole tile,
+          // so the last tile's stores need no bounds test; rows from N on
+          // get 0.  y is read at min(row, N_pad - 1), in bounds.
+          const int64_t rb = n_begin + t * kWsTile + wave * kWsWaveRows + 4 * (lane >> 4);
+          const int64_t row0 = n_begin + t * kWsTile;
+          bf16_t* rt = Rt + (row0 >> 5) * ldr * 32;
+          const uint32_t c0 = (uint32_t)(tm * kWsBM + l16) * 32u;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int64_t r0 = rb + 16 * j;
+            const int64_t left = n_end - r0;
+            float yv[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) yv[v] = y[min(r0 + v, N_pad - 1)];
+            const int nl = (int)(r0 - row0);                       // row within the tile
+            const uint32_t ro = (uint32_t)(nl >> 5) * (uint32_t)ldr * 32u + (uint32_t)(nl & 31) + c0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              float s32 = 0.0f;
+              float rv[4];
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                const float eta = acc[i][j][v];
+                // softplus and sigmoid from one t = exp(-|eta|)
+                const float tt = __builtin_amdgcn_exp2f(-fabsf(eta) * 1.4426950408889634f);
+                const float big = __builtin_amdgcn_logf(1.0f + tt) * 0.6931471805599453f;
+                const float small = tt * (1.0f - 0.5f * tt);
+                const float sp = fmaxf(eta, 0.0f) + (tt < 2.44140625e-4f ? small : big);
+                const float inv = fast_rcp(1.0f + tt);
+                const float sig = eta >= 0.0f ? inv : tt * inv;
+                s32 -= v < left ? sp : 0.0f;
+                rv[v] = v < left ? yv[v] - sig : 0.0f;
+              }
+              lls[i][0] += (double)s32;
+              uint2 pk;
+              pk.x = (uint32_t)f32_to_bf16(rv[0]) | ((uint32_t)f32_to_bf16(rv[1]) << 16);
+              pk.y = (uint32_t)f32_to_bf16(rv[2]) | ((uint32_t)f32_to_bf16(rv[3]) << 16);
+              *(uint2*)(rt + ro + (uint32_t)(16 * i) * 32u) = pk;
+            }
           }
+        }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -1316,23 +1411,38 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
       ks0 = kn;
     }
   }
-  // lanes with equal lane >> 4 hold the same chains at different rows
+  if (!WR) {
+    // lanes with equal lane >> 4 hold the same chains at different rows
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      double x = lls[i][v];
+      for (int v = 0; v < 4; ++v) {
+        double x = lls[i][v];
 #pragma unroll
-      for (int off = 1; off < 16; off <<= 1) x += __shfl_xor(x, off, 64);
-      lls[i][v] = x;
+        for (int off = 1; off < 16; off <<= 1) x += __shfl_xor(x, off, 64);
+        lls[i][v] = x;
+      }
+  } else {
+    // lanes with equal l16 hold the same chain at different rows
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double x = lls[i][0];
+      x += __shfl_xor(x, 16, 64);
+      x += __shfl_xor(x, 32, 64);
+      lls[i][0] = x;
     }
+  }
   __syncthreads();   // the W tiles are no longer read: reuse LDS for the wave sums
   double* red = reinterpret_cast<double*>(lds);   // [8 waves][64 chains]
-  if (l16 == 0) {
+  if (!WR && l16 == 0) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int v = 0; v < 4; ++v) red[wave * kWsBM + 16 * i + (lane >> 4) * 4 + v] = lls[i][v];
+  }
+  if (WR && lane < 16) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[wave * kWsBM + 16 * i + l16] = lls[i][0];
   }
   __syncthreads();
   if (tid < kWsBM) {
@@ -1354,7 +1464,7 @@ static bool ws_forward_ok(const edhmc_ctx* ctx) {
          ctx->N >= 16 * kWsTile;
 }
 
-static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st) {
+static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st, bool wr) {
   const int m_tiles = ctx->C_pad / kWsBM;
   // about one workgroup per CU: split the rows of each chain tile into parts
   int parts = std::max(1, 256 / m_tiles);
@@ -1370,26 +1480,33 @@ static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st) {
   // an eighth of them over all parts
   const int mt_per_xcd = part_per_xcd ? m_tiles / (8 / parts) : m_tiles / 8;
   const int grid = m_tiles * parts;
-  ProfScope ps(ctx, PK_GEMM_LL, st);
-  hipLaunchKernelGGL(gemm_ll_ws_kernel, dim3(grid), dim3(kWsThreads), 0, st,
-                     (const bf16_t*)ctx->z_lp.p, (int64_t)ctx->C_pad * ctx->D_pad, ctx->D_pad,
-                     (const bf16_t*)ctx->Xr.p, ctx->N, ctx->N_pad, rows_per_part, parts,
-                     mt_per_xcd, part_per_xcd ? 1 : 0, (double*)ctx->llpart.p,
-                     (int64_t)ctx->C_pad);
+  {
+    ProfScope ps(ctx, wr ? PK_GEMM_GRAD : PK_GEMM_LL, st);
+    if (wr)
+      hipLaunchKernelGGL((gemm_ll_ws_kernel<true, kWsPreGrad>), dim3(grid), dim3(kWsThreads), 0, st,
+                         (const bf16_t*)ctx->z_lp.p, (int64_t)ctx->C_pad * ctx->D_pad,
+                         ctx->D_pad, (const bf16_t*)ctx->Xr.p, ctx->N, ctx->N_pad,
+                         rows_per_part, parts, mt_per_xcd, part_per_xcd ? 1 : 0,
+                         (double*)ctx->llpart.p, (int64_t)ctx->C_pad, (const float*)ctx->y.p,
+                         (bf16_t*)ctx->Rt.p, (int64_t)ctx->C_pad);
+    else
+      hipLaunchKernelGGL((gemm_ll_ws_kernel<false, kWsPre>), dim3(grid), dim3(kWsThreads), 0, st,
+                         (const bf16_t*)ctx->z_lp.p, (int64_t)ctx->C_pad * ctx->D_pad,
+                         ctx->D_pad, (const bf16_t*)ctx->Xr.p, ctx->N, ctx->N_pad,
+                         rows_per_part, parts, mt_per_xcd, part_per_xcd ? 1 : 0,
+                         (double*)ctx->llpart.p, (int64_t)ctx->C_pad, nullptr, nullptr,
+                         (int64_t)0);
+  }
   EDHMC_TRY(check_launch(ctx, "gemm_ll_ws"));
   return ll_reduce(ctx, parts, st);
 }
 
 // Forward only (log-likelihood partials, no residuals) at z_lp, exact form.
 int glm_forward_ll(edhmc_ctx* ctx, hipStream_t st) {
-  if (ws_forward_ok(ctx)) EDHMC_TRY(launch_ws_forward(ctx, st));
+  if (ws_forward_ok(ctx)) EDHMC_TRY(launch_ws_forward(ctx, st, false));
   else if (ctx->cdt == EDHMC_BF16) EDHMC_TRY((launch_gemm1<bf16_t, CfgP2, 2, false, true>(ctx, st)));
   else EDHMC_TRY((launch_gemm1<float, Cfg1, 1, false, true>(ctx, st)));
-  if (ctx->lik != EDHMC_LIK_BERNOULLI) return EDHMC_OK;
-  hipLaunchKernelGGL(ll_linear_kernel, dim3(ctx->C), dim3(256), 0, st, (const void*)ctx->z_lp.p,
-                     ctx->cdt == EDHMC_BF16 ? 1 : 0, ctx->D, ctx->D_pad, ctx->C_pad,
-                     (const double*)ctx->xty.p, (double*)ctx->ll.p);
-  return check_launch(ctx, "ll_linear");
+  return ll_linear(ctx, st);
 }
 
 // Recompute the carried log joint at z with the forward-only pass (the
@@ -1480,7 +1597,9 @@ static int ensure_chains(edhmc_ctx* ctx, int C) {
   EDHMC_TRY(dev_alloc(ctx, ctx->logp, (size_t)C * 8));
   EDHMC_TRY(dev_alloc(ctx, ctx->K0, (size_t)C * 8));
   EDHMC_TRY(dev_alloc(ctx, ctx->ndiv, (size_t)C * 4));
-  EDHMC_TRY(dev_alloc(ctx, ctx->Rt, (size_t)C_pad * ctx->N_pad * eb, false));
+  // rows up to a whole W-stationary tile: its residual epilogue stores the
+  // last tile's rows past N_pad (zeros) without a bounds test
+  EDHMC_TRY(dev_alloc(ctx, ctx->Rt, (size_t)C_pad * round_up(ctx->N_pad, kWsTile) * eb, false));
   // split-K for GEMM2: aim for >= 2048 workgroups, >= 8 stages per split.
   const int bn2 = (ctx->D_pad % Cfg1::BN == 0) ? Cfg1::BN : Cfg2S::BN;
   const int tiles2 = (C_pad / 128) * (ctx->D_pad / bn2);
