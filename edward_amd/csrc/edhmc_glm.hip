@@ -106,10 +106,19 @@ struct EpiGLM {
   int64_t goff_ld = 0;
   const int* gidx = nullptr;
 
-  // The residual tile goes through LDS when it fits beside the ll scratch:
-  // lanes write it row-major ([m][n], padded rows), then each thread stores
-  // whole 16-byte pieces of the K-blocked layout, in which the BM rows of
-  // one 64-byte K block are contiguous -- instead of 2-byte scattered stores.
+  // The residual tile goes through LDS: lanes write it row-major ([m][n],
+  // padded rows), then each thread stores whole 16-byte pieces of the
+  // K-blocked layout, in which the BM rows of one 64-byte K block are
+  // contiguous -- instead of 2- or 4-byte scattered stores.  lds_bytes asks
+  // the kernel for room for it beside the ll scratch and the staged group
+  // intercepts.
+  static constexpr int KG = 8;   // group intercepts staged per row tile
+  template <typename Cfg>
+  static constexpr int lds_bytes() {
+    return WR ? ((Cfg::BM * (Cfg::BN * (int)sizeof(T) + 16) + 15) / 16 * 16 +
+                 Cfg::kWavesN * Cfg::BM * 8 + Cfg::BM * KG * 4)
+              : 0;
+  }
   template <typename Cfg>
   __device__ void apply(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p) const {
     constexpr int WM = Cfg::kWM, WN = Cfg::kWN;
@@ -136,7 +145,6 @@ struct EpiGLM {
     // groups g_lo..g_hi; when that range is short, the tile's intercepts
     // a[m][g_lo..g_hi] are staged in LDS once (one coalesced pass) instead of
     // one scattered global load per accumulator element.
-    constexpr int KG = 8;
     float* gl = reinterpret_cast<float*>(p.smem + (stage ? RTILE : 0) + LLB);
     int g_lo = 0;
     bool g_lds = false;

@@ -27,6 +27,8 @@
 // tile (mode 1) or the same K split (mode 2) land on one XCD and share its L2.
 #pragma once
 
+#include <type_traits>
+
 #include "common.h"
 
 namespace edhmc {
@@ -173,6 +175,17 @@ struct EpiPos {
 #ifndef EDHMC_GEMM_WPE
 #define EDHMC_GEMM_WPE 2
 #endif
+
+// LDS bytes an epilogue asks for (its static lds_bytes<Cfg>(), 0 without
+// one); the kernel reserves the larger of that and its main loop's buffers.
+template <typename Epi, typename Cfg, typename = void>
+struct EpiLds {
+  static constexpr int value = 0;
+};
+template <typename Epi, typename Cfg>
+struct EpiLds<Epi, Cfg, std::void_t<decltype(Epi::template lds_bytes<Cfg>())>> {
+  static constexpr int value = Epi::template lds_bytes<Cfg>();
+};
 template <typename T, typename Cfg, int A_PLANES, typename Epi>
 __global__ void __launch_bounds__(Cfg::kThreads) __attribute__((amdgpu_waves_per_eu(1, EDHMC_GEMM_WPE)))
 gemm_nt_kernel(GemmShape g, Epi epi) {
@@ -185,7 +198,8 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
   constexpr int CHUNKS = ROWS * CPR;
   constexpr int PER_T = (CHUNKS + NT - 1) / NT;
   constexpr int BUF = ROWS * Cfg::RS;
-  __shared__ __align__(16) char smem[2 * BUF];
+  constexpr int SMEM = 2 * BUF > EpiLds<Epi, Cfg>::value ? 2 * BUF : EpiLds<Epi, Cfg>::value;
+  __shared__ __align__(16) char smem[SMEM];
 
   int tm, tn, ks;
   if (!map_tile(g, blockIdx.x, tm, tn, ks)) return;
@@ -388,7 +402,7 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
   pos.n0 = tn * BN + wn * 32 * WN;
   pos.lane = lane; pos.wm = wm; pos.wn = wn; pos.tm = tm; pos.tn = tn; pos.ks = ks;
   pos.smem = smem;
-  pos.smem_bytes = 2 * BUF;
+  pos.smem_bytes = SMEM;
   epi.template apply<Cfg>(acc, pos);
 }
 
