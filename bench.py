@@ -151,8 +151,9 @@ def cpu_baseline(wl, budget_s, ess_per_transition_plain):
 
 
 def bench_hier(args):
-  """Plain fp32 HMC on the hierarchical model: transitions/s and the
-  gradient GEMM's share, with the device profiler over the timed region."""
+  """Plain fp32 HMC on the hierarchical model: transitions/s over an
+  unprofiled timed region, then the kernel classes' times from a few more
+  transitions with every class bracketed by events."""
   import torch
   from edward_amd import workloads as W
   from edward_amd.hier import HierEngine
@@ -168,14 +169,18 @@ def bench_hier(args):
   torch.cuda.synchronize(dev)
   n_acc.zero_()
   K = args.steps
-  eng.profile(True)
   t0 = time.time()
   for t in range(K):
     eng.transition(eps, L, seed=args.seed, counter=args.warmup + t, n_accept=n_acc)
   torch.cuda.synchronize(dev)
   secs = time.time() - t0
-  prof = eng.profile_read()
   acc = float(n_acc.double().mean().item()) / K
+  n_break = 3
+  eng.profile(True)
+  for t in range(n_break):
+    eng.transition(eps, L, seed=args.seed, counter=args.warmup + K + t)
+  prof = eng.profile_read()
+  eng.profile(False)
   # one gradient pass: eta = X w (2NDC) and X^T R (2NDC) flops
   flops = 4.0 * wl.N * wl.D * C
   g_ms, g_n = prof['gemm_grad']
@@ -193,7 +198,9 @@ def bench_hier(args):
       'config': {'workload': 'hier_logreg G=%d N=%d D=%d chains=%d eps=%g L=%d'
                              % (wl.G, wl.N, wl.D, C, eps, L)},
       'accept': acc,
-      'profile_ms_per_transition': {k: v[0] / K for k, v in prof.items()},
+      'profile_ms_per_transition': {k: v[0] / n_break for k, v in prof.items()},
+      'profile_from': '%d transitions after the timed region, every kernel class bracketed '
+                      'by events' % n_break,
       'roofline': {'bound': 'mfma', 'kernel': 'gemm_grad+gemm_xtr (one gradient pass)',
                    'achieved': achieved / 1e12, 'peak': PEAK['fp32'] / 1e12,
                    'unit': 'TFLOP/s', 'frac': achieved / PEAK['fp32'], 'traffic': None},
