@@ -1128,7 +1128,8 @@ static int launch_gemm2(edhmc_ctx* ctx, hipStream_t st) {
 // One gradient evaluation at z_lp.  split: use both bf16 planes; ll: write the
 // log-likelihood partials.
 static bool ws_forward_ok(const edhmc_ctx* ctx);
-static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st, bool wr);
+static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st, bool wr, int planes = 2,
+                             bool ll = true);
 static bool ws_grad_ok(const edhmc_ctx* ctx) {
 #ifdef EDHMC_NO_WS_GRAD
   return false;
@@ -1161,6 +1162,7 @@ static int grad_eval(edhmc_ctx* ctx, bool split, bool ll, hipStream_t st) {
       }
     } else {
       if (ll) EDHMC_TRY((launch_gemm1<bf16_t, Cfg1, 1, true, true>(ctx, st)));
+      else if (ws_grad_ok(ctx)) EDHMC_TRY(launch_ws_forward(ctx, st, true, 1, false));
       else EDHMC_TRY((launch_gemm1<bf16_t, Cfg1, 1, true, false>(ctx, st)));
     }
     return launch_gemm2<bf16_t>(ctx, st);
@@ -1215,6 +1217,10 @@ constexpr int kWsPre = EDHMC_WS_PRE;       // 64-byte k steps of X in flight per
 #define EDHMC_WS_PRE_GRAD 2
 #endif
 constexpr int kWsPreGrad = EDHMC_WS_PRE_GRAD;   // the same with the residual epilogue
+#ifndef EDHMC_WS_PRE_GRAD1
+#define EDHMC_WS_PRE_GRAD1 2
+#endif
+constexpr int kWsPreGrad1 = EDHMC_WS_PRE_GRAD1; // ... on one plane without the log-likelihood
 constexpr int kWsMaxDPad = 512;
 #ifndef EDHMC_WS_PAD
 #define EDHMC_WS_PAD 16
@@ -1234,7 +1240,9 @@ __device__ inline void mfma16(f32x4v_t& acc, const uint4& a, const uint4& b) {
 // (0 for the padded rows N .. N_pad), beside the log-likelihood partials.
 // PRE: 64-byte k steps of X in flight per wave (fewer with the residual
 // epilogue, whose registers they would otherwise crowd out).
-template <bool WR, int PRE>
+// PLANES: 2 (hi and lo planes of W, the exact log joint) or 1 (hi plane
+// only: the one-plane inner leapfrog gradients); LL: sum -softplus(eta).
+template <bool WR, int PLANES, bool LL, int PRE>
 __global__ void __launch_bounds__(kWsThreads) __attribute__((amdgpu_waves_per_eu(1, 2)))
 gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
                   const bf16_t* __restrict__ X, int64_t N, int64_t N_pad, int64_t rows_per_part,
@@ -1260,7 +1268,7 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
   const int rsb = 2 * D_pad + kWsPad;
   // resident W: LDS row p * 64 + c = plane p of chain 64 tm + c
   const int cpr = D_pad / 8;                 // 16-byte chunks per row
-  for (int q = tid; q < 2 * kWsBM * cpr; q += kWsThreads) {
+  for (int q = tid; q < PLANES * kWsBM * cpr; q += kWsThreads) {
     const int row = q / cpr, ch = q - row * cpr;
     const int pl = row / kWsBM, c = row - pl * kWsBM;
     const uint4 v = *(const uint4*)(Wp + pl * plane_elems + (int64_t)(tm * kWsBM + c) * D_pad +
@@ -1329,7 +1337,7 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
       for (int u = 0; u < PRE; ++u) {
         const int ks = ks0 + u;
 #pragma unroll
-        for (int pl = 0; pl < 2; ++pl) {
+        for (int pl = 0; pl < PLANES; ++pl) {
           uint4 wf[4];
 #pragma unroll
           for (int i = 0; i < 4; ++i)
@@ -1394,15 +1402,17 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
                 const float eta = acc[i][j][v];
                 // softplus and sigmoid from one t = exp(-|eta|)
                 const float tt = __builtin_amdgcn_exp2f(-fabsf(eta) * 1.4426950408889634f);
-                const float big = __builtin_amdgcn_logf(1.0f + tt) * 0.6931471805599453f;
-                const float small = tt * (1.0f - 0.5f * tt);
-                const float sp = fmaxf(eta, 0.0f) + (tt < 2.44140625e-4f ? small : big);
+                if (LL) {
+                  const float big = __builtin_amdgcn_logf(1.0f + tt) * 0.6931471805599453f;
+                  const float small = tt * (1.0f - 0.5f * tt);
+                  const float sp = fmaxf(eta, 0.0f) + (tt < 2.44140625e-4f ? small : big);
+                  s32 -= v < left ? sp : 0.0f;
+                }
                 const float inv = fast_rcp(1.0f + tt);
                 const float sig = eta >= 0.0f ? inv : tt * inv;
-                s32 -= v < left ? sp : 0.0f;
                 rv[v] = v < left ? yv[v] - sig : 0.0f;
               }
-              lls[i][0] += (double)s32;
+              if (LL) lls[i][0] += (double)s32;
               uint2 pk;
               pk.x = (uint32_t)f32_to_bf16(rv[0]) | ((uint32_t)f32_to_bf16(rv[1]) << 16);
               pk.y = (uint32_t)f32_to_bf16(rv[2]) | ((uint32_t)f32_to_bf16(rv[3]) << 16);
@@ -1419,6 +1429,7 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
       ks0 = kn;
     }
   }
+  if (!LL) return;   // uniform across the workgroup: no barrier below is skipped by part of it
   if (!WR) {
     // lanes with equal lane >> 4 hold the same chains at different rows
 #pragma unroll
@@ -1472,7 +1483,7 @@ static bool ws_forward_ok(const edhmc_ctx* ctx) {
          ctx->N >= 16 * kWsTile;
 }
 
-static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st, bool wr) {
+static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st, bool wr, int planes, bool ll) {
   const int m_tiles = ctx->C_pad / kWsBM;
   // about one workgroup per CU: split the rows of each chain tile into parts
   int parts = std::max(1, 256 / m_tiles);
@@ -1490,23 +1501,33 @@ static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st, bool wr) {
   const int grid = m_tiles * parts;
   {
     ProfScope ps(ctx, wr ? PK_GEMM_GRAD : PK_GEMM_LL, st);
-    if (wr)
-      hipLaunchKernelGGL((gemm_ll_ws_kernel<true, kWsPreGrad>), dim3(grid), dim3(kWsThreads), 0, st,
-                         (const bf16_t*)ctx->z_lp.p, (int64_t)ctx->C_pad * ctx->D_pad,
-                         ctx->D_pad, (const bf16_t*)ctx->Xr.p, ctx->N, ctx->N_pad,
-                         rows_per_part, parts, mt_per_xcd, part_per_xcd ? 1 : 0,
-                         (double*)ctx->llpart.p, (int64_t)ctx->C_pad, (const float*)ctx->y.p,
-                         (bf16_t*)ctx->Rt.p, (int64_t)ctx->C_pad);
-    else
-      hipLaunchKernelGGL((gemm_ll_ws_kernel<false, kWsPre>), dim3(grid), dim3(kWsThreads), 0, st,
+    if (wr && planes == 1 && !ll)
+      hipLaunchKernelGGL((gemm_ll_ws_kernel<true, 1, false, kWsPreGrad1>), dim3(grid),
+                         dim3(kWsThreads), 0, st, (const bf16_t*)ctx->z_lp.p,
+                         (int64_t)ctx->C_pad * ctx->D_pad, ctx->D_pad, (const bf16_t*)ctx->Xr.p,
+                         ctx->N, ctx->N_pad, rows_per_part, parts, mt_per_xcd,
+                         part_per_xcd ? 1 : 0, (double*)ctx->llpart.p, (int64_t)ctx->C_pad,
+                         (const float*)ctx->y.p, (bf16_t*)ctx->Rt.p, (int64_t)ctx->C_pad);
+    else if (wr && planes == 2 && ll)
+      hipLaunchKernelGGL((gemm_ll_ws_kernel<true, 2, true, kWsPreGrad>), dim3(grid),
+                         dim3(kWsThreads), 0, st, (const bf16_t*)ctx->z_lp.p,
+                         (int64_t)ctx->C_pad * ctx->D_pad, ctx->D_pad, (const bf16_t*)ctx->Xr.p,
+                         ctx->N, ctx->N_pad, rows_per_part, parts, mt_per_xcd,
+                         part_per_xcd ? 1 : 0, (double*)ctx->llpart.p, (int64_t)ctx->C_pad,
+                         (const float*)ctx->y.p, (bf16_t*)ctx->Rt.p, (int64_t)ctx->C_pad);
+    else if (!wr && planes == 2 && ll)
+      hipLaunchKernelGGL((gemm_ll_ws_kernel<false, 2, true, kWsPre>), dim3(grid), dim3(kWsThreads), 0, st,
                          (const bf16_t*)ctx->z_lp.p, (int64_t)ctx->C_pad * ctx->D_pad,
                          ctx->D_pad, (const bf16_t*)ctx->Xr.p, ctx->N, ctx->N_pad,
                          rows_per_part, parts, mt_per_xcd, part_per_xcd ? 1 : 0,
                          (double*)ctx->llpart.p, (int64_t)ctx->C_pad, nullptr, nullptr,
                          (int64_t)0);
+    else
+      return set_err(ctx, EDHMC_EUNSUPPORTED, "W-stationary pass: no variant for %d planes",
+                     planes);
   }
   EDHMC_TRY(check_launch(ctx, "gemm_ll_ws"));
-  return ll_reduce(ctx, parts, st);
+  return ll ? ll_reduce(ctx, parts, st) : EDHMC_OK;
 }
 
 // Forward only (log-likelihood partials, no residuals) at z_lp, exact form.

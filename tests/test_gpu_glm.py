@@ -328,3 +328,40 @@ def test_forward_only_logp_chain_stationary_path(N, D, C):
   lpg, g = eng.logp_grad(W0)
   np.testing.assert_allclose(lp, lpg.cpu().numpy(), rtol=2e-6)
   _check_logp_grad(lpg.cpu()[sel], g.cpu()[sel], lp_ref, g_ref, bf16=True)
+
+
+def _bf16_round(x):
+  u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+  u = (u + 0x7fff + ((u >> 16) & 1)) >> 16 << 16
+  return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def test_plain_bf16_transition_on_the_chain_stationary_path():
+  """A plain bf16 transition at a size the W-stationary kernel takes (its
+  one-plane residual pass for the inner leapfrog steps, its two-plane pass
+  with the log-likelihood for the first and last gradients): end positions of
+  chains spread over the chain tiles against the oracle's leapfrog with the
+  inner gradients at bf16-rounded positions (the hi plane), within 1e-3
+  relative per chain (bf16 residuals; measured ~1e-4)."""
+  from edward_amd import workloads as W
+  N, D, C, eps, L = 20000, 128, 512, 0.002, 8
+  wl = W.logreg(N, D, C, eps, L, 10, seed=41, bf16=True)
+  eng = _engine(wl, precision='bf16')
+  m = _oracle(wl)
+  rng = np.random.default_rng(3)
+  W0 = (wl.w_true[None, :] + 0.01 * rng.normal(size=(C, D))).astype(np.float32)
+  r0 = rng.normal(size=(C, D)).astype(np.float32)
+  eng.set_state(W0)
+  out = torch.empty(C, D, device='cuda')
+  eng.transition(eps, L, momenta=r0, uniforms=np.zeros(C), sample_out=out)
+  sel = np.array([0, 77, 130, 257, 390, 511])
+  z, r = W0[sel].astype(np.float64), r0[sel].astype(np.float64)
+  _, g = m.logp_grad(z)
+  for k in range(L):
+    r = r + 0.5 * eps * g
+    z = z + eps * r
+    _, g = m.logp_grad(z if k == L - 1 else _bf16_round(z))
+    r = r + 0.5 * eps * g
+  zd = out.cpu().numpy()[sel].astype(np.float64)
+  err = np.linalg.norm(zd - z, axis=1) / np.linalg.norm(z, axis=1)
+  assert err.max() < 1e-3, err
