@@ -79,7 +79,7 @@ def parse():
                   help='surrogate training pairs: exact gradients at Laplace draws (with '
                        '--init laplace; no burn-in transitions), or recorded along burn-in '
                        'trajectories')
-  ap.add_argument('--n-pairs', type=int, default=229376,
+  ap.add_argument('--n-pairs', type=int, default=114688,
                   help='training pairs drawn from the Laplace approximation')
   ap.add_argument('--burn-in-adapt', type=int, default=1,
                   help='1: the burn-in step-size centre follows the acceptance rate')
@@ -92,7 +92,7 @@ def parse():
                        "of HBM traffic profiles; 'forward': --steps exact forward passes "
                        "(the gemm_ll kernel alone) on the large workload, for profiling")
   ap.add_argument('--hidden', type=int, default=1024)
-  ap.add_argument('--epochs', type=int, default=12)
+  ap.add_argument('--epochs', type=int, default=32)
   ap.add_argument('--lr', type=float, default=3e-3)
   ap.add_argument('--rec-stride', type=int, default=4)
   ap.add_argument('--rec-transitions', type=int, default=8)
