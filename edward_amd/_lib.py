@@ -63,6 +63,8 @@ SIGNATURES = {
     'edhmc_moments_enable': (ctypes.c_int, [_c_p, _i32]),
     'edhmc_moments_get': (ctypes.c_int, [_c_p, _c_p, _c_p, ctypes.POINTER(_i64), _c_p]),
     'edhmc_divergences': (ctypes.c_int, [_c_p, _c_p, _c_p]),
+    'edhmc_last_ratio': (ctypes.c_int, [_c_p, _c_p, _c_p]),
+    'edhmc_get_momentum': (ctypes.c_int, [_c_p, _c_p, _c_p]),
     'edhmc_sample_moments': (ctypes.c_int, [_c_p, _c_p, _i32, _i64, _i64, _i64, _c_p, _c_p,
                                             _c_p]),
     'edhmc_record_pairs': (ctypes.c_int, [_c_p, _c_p, _c_p, _i64, _i32]),

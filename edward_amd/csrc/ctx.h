@@ -82,6 +82,7 @@ struct edhmc_ctx {
   bool state_valid = false;
   bool g_valid = false;   // g holds the exact gradient at z
   edhmc::DevBuf z, r, g, z0, g0, z_lp, logp, K0, flag, ndiv;
+  edhmc::DevBuf ratio;   // [C] FP64 Metropolis log ratio of the last HMC transition
   // ---- gradient scratch ----
   edhmc::DevBuf Rt, Gpart, llpart, ll;
   int ksplit = 1;
@@ -99,7 +100,6 @@ struct edhmc_ctx {
   edhmc::DevBuf tmp_count, stats_part;
   edhmc::DevBuf pred_col;   // posterior-predictive column partials [m_tiles][N_pad]
   edhmc::DevBuf ll_split;   // log-likelihood range sums [16][C_pad] (fixed-order reduction)
-  edhmc::DevBuf xty, xty_part;   // X^T y (FP64, Bernoulli) and its row-chunk partials
   edhmc::DevBuf newton;          // scratch of edhmc_newton_terms
 
   edhmc::Profiler prof;

@@ -164,10 +164,9 @@ struct EpiGLM {
         }
       }
     }
-    // Forward-only Bernoulli pass: the linear part sum_n y_n eta_n = w . X^T y
-    // is added per chain in FP64 after the reduction (ll_linear_kernel), so the
-    // epilogue sums only -softplus(eta) (fp32 over the lane's WN columns, FP64
-    // across them).
+    // Forward-only Bernoulli pass: the epilogue sums -(softplus(eta) - y eta)
+    // in fp32 over the lane's WN columns (non-negative terms, bern_nll_fast),
+    // FP64 across them.
     const bool sp_only = LL && !WR && !goff && lp.lik == LIK_BERNOULLI;
 #pragma unroll
     for (int i = 0; i < WM; ++i)
@@ -180,8 +179,8 @@ struct EpiGLM {
           float s32 = 0.0f;
 #pragma unroll
           for (int j = 0; j < WN; ++j) {
-            const float sp = softplus_fast(acc[i][j][r]);   // unconditional: no branch
-            s32 -= valid[j] ? sp : 0.0f;
+            const float nl = bern_nll_fast(acc[i][j][r], yv[j]);   // unconditional: no branch
+            s32 -= valid[j] ? nl : 0.0f;
           }
           llsum = (double)s32;
         } else {
@@ -434,55 +433,6 @@ ll_final_kernel(const double* __restrict__ split, int C_pad, double* __restrict_
   ll[c] = s;
 }
 
-// X^T y in FP64 over the compute-precision rows (the values the GEMMs use):
-// pass 1 per (row chunk, column), pass 2 the chunks in a fixed order.
-template <typename T>
-__global__ void __launch_bounds__(256)
-xty_part_kernel(const T* __restrict__ Xr, const float* __restrict__ y, int64_t N, int D_pad,
-                int chunks, double* __restrict__ part) {
-  const int d = blockIdx.x * 256 + threadIdx.x;
-  if (d >= D_pad) return;
-  const int64_t per = (N + chunks - 1) / chunks;
-  const int64_t b = (int64_t)blockIdx.y * per, e = min(N, b + per);
-  double s = 0.0;
-  for (int64_t n = b; n < e; ++n)
-    s += (double)y[n] * (double)ElemOps<T>::to_f32(Xr[n * D_pad + d]);
-  part[(int64_t)blockIdx.y * D_pad + d] = s;
-}
-
-__global__ void __launch_bounds__(256)
-xty_final_kernel(const double* __restrict__ part, int chunks, int D_pad, double* __restrict__ xty) {
-  const int d = blockIdx.x * 256 + threadIdx.x;
-  if (d >= D_pad) return;
-  double s = 0.0;
-  for (int k = 0; k < chunks; ++k) s += part[(int64_t)k * D_pad + d];
-  xty[d] = s;
-}
-
-// ll[c] += w_c . X^T y (FP64), w_c read back from the GEMM's position planes
-// (hi + lo for bf16): completes the forward-only Bernoulli log-likelihood,
-// whose epilogue summed -softplus(eta) only.
-__global__ void __launch_bounds__(256)
-ll_linear_kernel(const void* __restrict__ z_lp, int bf16, int D, int D_pad, int C_pad,
-                 const double* __restrict__ xty, double* __restrict__ ll) {
-  __shared__ double red[4];
-  const int c = blockIdx.x;
-  double s = 0.0;
-  for (int d = threadIdx.x; d < D; d += 256) {
-    const int64_t idx = (int64_t)c * D_pad + d;
-    double w;
-    if (bf16) {
-      const bf16_t* p = (const bf16_t*)z_lp;
-      w = (double)bf16_to_f32(p[idx]) + (double)bf16_to_f32(p[idx + (int64_t)C_pad * D_pad]);
-    } else {
-      w = (double)((const float*)z_lp)[idx];
-    }
-    s += w * xty[d];
-  }
-  s = block_sum_f64<256>(s, red);
-  if (threadIdx.x == 0) ll[c] += s;
-}
-
 __global__ void pad_copy_kernel(const float* __restrict__ src, int64_t n, float* __restrict__ dst,
                                 int64_t n_pad, float fill, int* nonfinite) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -515,6 +465,7 @@ struct ChainArgs {
   float lik_weight;         // multiplies the likelihood (scale of the SG samplers; 1 for HMC)
   const float* beta;        // per-chain inverse temperature of the MH ratio (null = 1)
   float* sample_out; int32_t* accepted_out; int32_t* n_accept; int32_t* ndiv;
+  double* ratio_out;        // per-chain log ratio of the HMC accept test (or null)
   double* mom_mean; double* mom_m2; int64_t mom_n;   // mom_n = count after update
   float* grad_out; double* logp_out;
   const float* W_in;        // set_state input [C, D]
@@ -671,6 +622,7 @@ __device__ inline void chain_accept(const ChainArgs& a, GradAt g_at) {
     if (a.accepted_out) a.accepted_out[c] = acc;
     if (a.n_accept) a.n_accept[c] += acc;
     if (a.ndiv && (nbad != 0.0 || !(ratio == ratio))) a.ndiv[c] += 1;
+    if (a.ratio_out) a.ratio_out[c] = ratio;
   }
   __syncthreads();
   const int acc = acc_flag;
@@ -1037,6 +989,7 @@ static ChainArgs chain_args(edhmc_ctx* ctx) {
   a.ll = (const double*)ctx->ll.p;
   a.lik_weight = ctx->lik_weight;
   a.ndiv = (int32_t*)ctx->ndiv.p;
+  a.ratio_out = (double*)ctx->ratio.p;
   return a;
 }
 
@@ -1137,24 +1090,13 @@ static bool ws_grad_ok(const edhmc_ctx* ctx) {
   return ws_forward_ok(ctx);
 }
 
-// Bernoulli: add w . X^T y (FP64, per chain) to the reduced log-likelihood of
-// a pass whose epilogue summed only -softplus(eta).
-static int ll_linear(edhmc_ctx* ctx, hipStream_t st) {
-  if (ctx->lik != EDHMC_LIK_BERNOULLI) return EDHMC_OK;
-  hipLaunchKernelGGL(ll_linear_kernel, dim3(ctx->C), dim3(256), 0, st, (const void*)ctx->z_lp.p,
-                     ctx->cdt == EDHMC_BF16 ? 1 : 0, ctx->D, ctx->D_pad, ctx->C_pad,
-                     (const double*)ctx->xty.p, (double*)ctx->ll.p);
-  return check_launch(ctx, "ll_linear");
-}
-
 static int grad_eval(edhmc_ctx* ctx, bool split, bool ll, hipStream_t st) {
   if (ctx->cdt == EDHMC_BF16) {
     if (split) {
       if (ll && ws_grad_ok(ctx)) {
-        // the W-stationary kernel writes the residuals beside its -softplus
-        // partials; the same log-likelihood formula as the forward-only pass
+        // the W-stationary kernel writes the residuals beside its
+        // log-likelihood partials (the forward-only pass's formula)
         EDHMC_TRY(launch_ws_forward(ctx, st, true));
-        EDHMC_TRY(ll_linear(ctx, st));
       } else if (ll) {
         EDHMC_TRY((launch_gemm1<bf16_t, CfgP2, 2, true, true>(ctx, st)));
       } else {
@@ -1197,7 +1139,7 @@ int glm_ensure_chains(edhmc_ctx* ctx, int C);
 
 // ---------------------------------------------------------------------------
 // W-stationary forward log-likelihood (Bernoulli, bf16 X, hi/lo W planes):
-// the sum over rows of -softplus(eta) for every chain, eta = W X^T.
+// the sum over rows of y eta - softplus(eta) for every chain, eta = W X^T.
 //
 // A workgroup (8 wave64s, one per CU) owns 64 chains and a contiguous range
 // of data rows.  The hi and lo planes of its 64 positions stay in LDS for
@@ -1211,7 +1153,7 @@ int glm_ensure_chains(edhmc_ctx* ctx, int C);
 // are in flight, the prefetch running on into the next tile while the
 // previous tile's epilogue runs; no LDS staging of X and no block barrier in
 // the main loop.  Per k step a wave issues 32 MFMAs (2 planes x 4 chain
-// blocks x 4 row blocks).  The epilogue of a tile adds -softplus over the
+// blocks x 4 row blocks).  The epilogue of a tile adds y eta - softplus(eta) over the
 // wave's valid rows into FP64 per-lane sums; at the end lanes, then waves,
 // are summed in a fixed order into llpart[part][chain].
 // ---------------------------------------------------------------------------
@@ -1251,7 +1193,8 @@ __device__ inline void mfma16(f32x4v_t& acc, const uint4& a, const uint4& b) {
 // PRE: 64-byte k steps of X in flight per wave (fewer with the residual
 // epilogue, whose registers they would otherwise crowd out).
 // PLANES: 2 (hi and lo planes of W, the exact log joint) or 1 (hi plane
-// only: the one-plane inner leapfrog gradients); LL: sum -softplus(eta).
+// only: the one-plane inner leapfrog gradients); LL: sum y eta - softplus(eta)
+// (bern_nll_fast's non-negative form).
 template <bool WR, int PLANES, bool LL, int PRE>
 __global__ void __launch_bounds__(kWsThreads) __attribute__((amdgpu_waves_per_eu(1, 2)))
 gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
@@ -1343,6 +1286,14 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
       const char* rp[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) rp[j] = row_ptr(tl, j) + kl * 64;
+      // forward: y of the wave's 64 rows of tile t, lane l holding row l (one
+      // coalesced load per k block, issued before this block's X loads so
+      // that the epilogue waits for it alone; padded rows read y = 0)
+      float yw = 0.0f;
+      if (!WR) {
+        const int64_t n = n_begin + t * kWsTile + wave * kWsWaveRows + lane;
+        yw = y[n < N_pad ? n : N_pad - 1];
+      }
 #pragma unroll
       for (int u = 0; u < PRE; ++u) {
         const int ks = ks0 + u;
@@ -1365,11 +1316,16 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
       }
       if (ks0 + PRE == KS) {
         if (!WR) {
-          // end of tile t: -softplus over the wave's valid rows (row j*16 + l16)
+          // end of tile t: the log-likelihood terms of the wave's valid rows
+          // (row j*16 + l16)
           const int64_t nb = n_begin + t * kWsTile + wave * kWsWaveRows + l16;
           bool valid[4];
+          float yv[4];
 #pragma unroll
-          for (int j = 0; j < 4; ++j) valid[j] = nb + 16 * j < n_end;
+          for (int j = 0; j < 4; ++j) {
+            valid[j] = nb + 16 * j < n_end;
+            yv[j] = __shfl(yw, 16 * j + l16, 64);
+          }
 #pragma unroll
           for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -1377,8 +1333,8 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
               float s32 = 0.0f;
 #pragma unroll
               for (int j = 0; j < 4; ++j) {
-                const float sp = softplus_fast(acc[i][j][v]);   // unconditional: no branch
-                s32 -= valid[j] ? sp : 0.0f;
+                const float nl = bern_nll_fast(acc[i][j][v], yv[j]);   // unconditional: no branch
+                s32 -= valid[j] ? nl : 0.0f;
               }
               lls[i][v] += (double)s32;
             }
@@ -1413,10 +1369,12 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
                 // softplus and sigmoid from one t = exp(-|eta|)
                 const float tt = __builtin_amdgcn_exp2f(-fabsf(eta) * 1.4426950408889634f);
                 if (LL) {
+                  // softplus(eta) - y eta as non-negative terms (bern_nll_fast)
                   const float big = __builtin_amdgcn_logf(1.0f + tt) * 0.6931471805599453f;
                   const float small = tt * (1.0f - 0.5f * tt);
-                  const float sp = fmaxf(eta, 0.0f) + (tt < 2.44140625e-4f ? small : big);
-                  s32 -= v < left ? sp : 0.0f;
+                  const float nl = (tt < 2.44140625e-4f ? small : big) +
+                                   yv[v] * fmaxf(-eta, 0.0f) + (1.0f - yv[v]) * fmaxf(eta, 0.0f);
+                  s32 -= v < left ? nl : 0.0f;
                 }
                 const float inv = fast_rcp(1.0f + tt);
                 const float sig = eta >= 0.0f ? inv : tt * inv;
@@ -1530,8 +1488,8 @@ static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st, bool wr, int planes
                          (const bf16_t*)ctx->z_lp.p, (int64_t)ctx->C_pad * ctx->D_pad,
                          ctx->D_pad, (const bf16_t*)ctx->Xr.p, ctx->N, ctx->N_pad,
                          rows_per_part, parts, mt_per_xcd, part_per_xcd ? 1 : 0,
-                         (double*)ctx->llpart.p, (int64_t)ctx->C_pad, nullptr, nullptr,
-                         (int64_t)0);
+                         (double*)ctx->llpart.p, (int64_t)ctx->C_pad, (const float*)ctx->y.p,
+                         nullptr, (int64_t)0);
     else
       return set_err(ctx, EDHMC_EUNSUPPORTED, "W-stationary pass: no variant for %d planes",
                      planes);
@@ -1545,7 +1503,7 @@ int glm_forward_ll(edhmc_ctx* ctx, hipStream_t st) {
   if (ws_forward_ok(ctx)) EDHMC_TRY(launch_ws_forward(ctx, st, false));
   else if (ctx->cdt == EDHMC_BF16) EDHMC_TRY((launch_gemm1<bf16_t, CfgP2, 2, false, true>(ctx, st)));
   else EDHMC_TRY((launch_gemm1<float, Cfg1, 1, false, true>(ctx, st)));
-  return ll_linear(ctx, st);
+  return EDHMC_OK;
 }
 
 // Recompute the carried log joint at z with the forward-only pass (the
@@ -1636,6 +1594,7 @@ static int ensure_chains(edhmc_ctx* ctx, int C) {
   EDHMC_TRY(dev_alloc(ctx, ctx->logp, (size_t)C * 8));
   EDHMC_TRY(dev_alloc(ctx, ctx->K0, (size_t)C * 8));
   EDHMC_TRY(dev_alloc(ctx, ctx->ndiv, (size_t)C * 4));
+  EDHMC_TRY(dev_alloc(ctx, ctx->ratio, (size_t)C * 8));
   // rows up to a whole W-stationary tile: its residual epilogue stores the
   // last tile's rows past N_pad (zeros) without a bounds test
   EDHMC_TRY(dev_alloc(ctx, ctx->Rt, (size_t)C_pad * round_up(ctx->N_pad, kWsTile) * eb, false));
@@ -1756,8 +1715,8 @@ edhmc_ctx::~edhmc_ctx() {
   if (sur) surrogate_destroy(sur);
   if (hier) hier_destroy(hier);
   DevBuf* bufs[] = {&Xr, &Xc, &y, &ploc, &pscale, &z, &r, &g, &z0, &g0, &z_lp, &logp, &K0,
-                    &flag, &ndiv, &Rt, &Gpart, &llpart, &ll, &mom_mean, &mom_m2, &tmp_count,
-                    &stats_part, &pred_col, &ll_split, &xty, &xty_part, &pkind, &plnorm,
+                    &flag, &ndiv, &ratio, &Rt, &Gpart, &llpart, &ll, &mom_mean, &mom_m2, &tmp_count,
+                    &stats_part, &pred_col, &ll_split, &pkind, &plnorm,
                     &newton};
   for (DevBuf* b : bufs) b->release();
 }
@@ -1862,23 +1821,6 @@ int edhmc_bind_glm(edhmc_ctx* ctx, const void* X, int32_t x_in_dtype, const floa
       hipStreamSynchronize(st) != hipSuccess)
     return set_err(ctx, EDHMC_EHIP, "bind: copy failed");
   if (nbad) return set_err(ctx, EDHMC_ENONFINITE, "data contains non-finite values (%d blocks)", nbad);
-  if (likelihood == EDHMC_LIK_BERNOULLI) {
-    // X^T y for the forward-only log-likelihood (its linear part)
-    const int chunks = (int)std::min<int64_t>(256, (N + 1023) / 1024);
-    EDHMC_TRY(dev_alloc(ctx, ctx->xty_part, (size_t)chunks * ctx->D_pad * 8, false));
-    EDHMC_TRY(dev_alloc(ctx, ctx->xty, (size_t)ctx->D_pad * 8, false));
-    const dim3 g1((ctx->D_pad + 255) / 256, chunks);
-    if (ctx->cdt == EDHMC_BF16)
-      hipLaunchKernelGGL(xty_part_kernel<bf16_t>, g1, dim3(256), 0, st, (const bf16_t*)ctx->Xr.p,
-                         (const float*)ctx->y.p, N, ctx->D_pad, chunks, (double*)ctx->xty_part.p);
-    else
-      hipLaunchKernelGGL(xty_part_kernel<float>, g1, dim3(256), 0, st, (const float*)ctx->Xr.p,
-                         (const float*)ctx->y.p, N, ctx->D_pad, chunks, (double*)ctx->xty_part.p);
-    EDHMC_TRY(check_launch(ctx, "xty_part"));
-    hipLaunchKernelGGL(xty_final_kernel, dim3((ctx->D_pad + 255) / 256), dim3(256), 0, st,
-                       (const double*)ctx->xty_part.p, chunks, ctx->D_pad, (double*)ctx->xty.p);
-    EDHMC_TRY(check_launch(ctx, "xty_final"));
-  }
   ctx->bound = true;
   return EDHMC_OK;
 }
@@ -1910,6 +1852,18 @@ int edhmc_get_state(edhmc_ctx* ctx, float* W_out, double* logp_out, float* grad_
                                  hipMemcpyDeviceToDevice, st) != hipSuccess)
     return set_err(ctx, EDHMC_EHIP, "get_state copy failed");
   return EDHMC_OK;
+}
+
+int edhmc_get_momentum(edhmc_ctx* ctx, float* r_out, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  if (!ctx->state_valid) return set_err(ctx, EDHMC_ESTATE, "no chain state");
+  if (!r_out) return set_err(ctx, EDHMC_EINVAL, "r_out is NULL");
+  const int64_t n = (int64_t)ctx->C * ctx->D;
+  hipLaunchKernelGGL(copy_out_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, (const float*)ctx->r.p, ctx->C, ctx->D, ctx->D_pad,
+                     r_out);
+  return check_launch(ctx, "copy_out");
 }
 
 int edhmc_logp_grad(edhmc_ctx* ctx, const float* W, int32_t C, float* grad_out, double* logp_out,
@@ -2318,6 +2272,16 @@ int edhmc_divergences(edhmc_ctx* ctx, int32_t* out, void* stream) {
   DeviceGuard dg(ctx->device);
   if (!ctx->ndiv.p || !out) return set_err(ctx, EDHMC_ESTATE, "no chains");
   if (hipMemcpyAsync(out, ctx->ndiv.p, (size_t)ctx->C * 4, hipMemcpyDeviceToDevice,
+                     (hipStream_t)stream) != hipSuccess)
+    return set_err(ctx, EDHMC_EHIP, "copy failed");
+  return EDHMC_OK;
+}
+
+int edhmc_last_ratio(edhmc_ctx* ctx, double* ratio_out, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  if (!ctx->ratio.p || !ratio_out) return set_err(ctx, EDHMC_ESTATE, "no chains");
+  if (hipMemcpyAsync(ratio_out, ctx->ratio.p, (size_t)ctx->C * 8, hipMemcpyDeviceToDevice,
                      (hipStream_t)stream) != hipSuccess)
     return set_err(ctx, EDHMC_EHIP, "copy failed");
   return EDHMC_OK;

@@ -102,6 +102,7 @@ struct HierArgs {
   float half, full;
   const float* r_fed; const double* u_fed; uint64_t seed, counter;
   float* sample_out; int32_t* accepted_out; int32_t* n_accept; int32_t* ndiv;
+  double* ratio_out;
   const float* W_in; float* grad_out; double* logp_out;
   // group-scale coordinate u: sigma = exp(u) (skind >> 4 == 0) or
   // softplus(u) (== 1); its prior is the family skind & 15 of transform.h
@@ -290,6 +291,7 @@ __global__ void __launch_bounds__(kHT) hier_last_kernel(HierArgs a) {
     if (a.accepted_out) a.accepted_out[c] = acc;
     if (a.n_accept) a.n_accept[c] += acc;
     if (a.ndiv && (nbad != 0.0 || !(ratio == ratio))) a.ndiv[c] += 1;
+    if (a.ratio_out) a.ratio_out[c] = ratio;
   }
   __syncthreads();
   const int acc = acc_flag;
@@ -320,6 +322,7 @@ static HierArgs hier_args(edhmc_ctx* ctx, HierModel* h) {
   a.logp = (double*)h->logp.p; a.K0 = (double*)h->K0.p;
   a.zlp = (float*)ctx->z_lp.p;
   a.ndiv = (int32_t*)ctx->ndiv.p;   // divergence counts share the GLM slot
+  a.ratio_out = (double*)ctx->ratio.p;   // so does the last transition's log ratio
   a.skind = h->skind; a.sp1 = h->sp1; a.sp2 = h->sp2; a.slnorm = h->slnorm;
   return a;
 }

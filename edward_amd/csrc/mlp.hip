@@ -66,6 +66,10 @@ struct Surrogate {
   DevBuf W1b, W2b, W3b, W2t, W3t;     // bf16 copies: forward [out x in], backward [in x out]
   DevBuf nrm;                          // in_shift, in_scale, out_shift, out_scale (each Dp)
   DevBuf x0, a1, h1, a2, h2, o;        // activations (batch Bp); h2 is [Bp x Kc] = [h2 | x]
+  // second [h2 | x] buffer of the surrogate leapfrog: step k's output layer
+  // reads buffer k & 1 while its epilogue writes the next input into the
+  // other one, so no GEMM operand changes while the GEMM runs
+  DevBuf h2x;
   DevBuf dO, da2, da1, x0t, h1t, h2t, dOt, da2t, da1t;
   DevBuf loss;                         // fp64 [1]
   DevBuf Wb, Gb;                       // a gathered training batch [Bmax x D] fp32 each
@@ -82,7 +86,7 @@ void surrogate_invalidate(Surrogate* s) {
 void surrogate_destroy(Surrogate* s) {
   if (!s) return;
   DevBuf* b[] = {&s->P, &s->Gr, &s->Am, &s->Av, &s->W1b, &s->W2b, &s->W3b, &s->W2t, &s->W3t,
-                 &s->nrm, &s->x0, &s->a1, &s->h1, &s->a2, &s->h2, &s->o, &s->dO, &s->da2,
+                 &s->nrm, &s->x0, &s->a1, &s->h1, &s->a2, &s->h2, &s->h2x, &s->o, &s->dO, &s->da2,
                  &s->da1, &s->x0t, &s->h1t, &s->h2t, &s->dOt, &s->da2t, &s->da1t, &s->loss,
                  &s->Wb, &s->Gb, &s->ghat, &s->ghat0};
   for (DevBuf* x : b) x->release();
@@ -202,8 +206,10 @@ struct EpiGradOut {
 
 // Leapfrog step fused into the output layer: ghat = surrogate gradient at z;
 // middle steps kick twice by eps/2 and drift, and write the next normalised
-// input; the last step kicks by eps/2 and writes the hi/lo position planes
-// for the exact log joint.
+// input (into x0 and into the x columns of the *other* [h2 | x] buffer: the
+// one this GEMM reads is an operand of sibling workgroups still running);
+// the last step kicks by eps/2 and writes the hi/lo position planes for the
+// exact log joint.
 struct EpiLeapfrog {
   const float* bias; const float* out_shift; const float* out_scale;
   const float* in_shift; const float* in_scale;
@@ -214,7 +220,7 @@ struct EpiLeapfrog {
   int eval_only;             // 1: store ghat only (no kick / drift)
   float* z; float* r; float* ghat;
   bf16_t* x0;                // next MLP input
-  bf16_t* xcat; int xcat_ld; // its copy in the output layer's [h2 | x] input
+  bf16_t* xcat; int xcat_ld; // its copy in the next step's [h2 | x] buffer
   bf16_t* zlp; int64_t zlp_plane; int zlp_ld;   // hi/lo planes (last step)
   template <typename Cfg>
   __device__ void apply(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p) const {
@@ -434,8 +440,10 @@ static int refresh_weights(edhmc_ctx* ctx, Surrogate* s, hipStream_t st) {
 }
 
 // Forward through the two hidden layers for the batch in s->x0 (Bp rows);
-// keep_a: store pre-activations for the backward pass.
-static int forward_hidden(edhmc_ctx* ctx, Surrogate* s, int Bp, bool keep_a, hipStream_t st) {
+// keep_a: store pre-activations for the backward pass; h2 goes into the
+// first Hp columns of h2_out ([Bp x Kc]; s->h2 unless given).
+static int forward_hidden(edhmc_ctx* ctx, Surrogate* s, int Bp, bool keep_a, hipStream_t st,
+                          bf16_t* h2_out = nullptr) {
   const float* P = (const float*)s->P.p;
   EpiBiasGelu e1;
   e1.bias = P + s->ob1;
@@ -447,7 +455,7 @@ static int forward_hidden(edhmc_ctx* ctx, Surrogate* s, int Bp, bool keep_a, hip
   EpiBiasGelu e2;
   e2.bias = P + s->ob2;
   e2.a_out = keep_a ? (float*)s->a2.p : nullptr;
-  e2.h_out = (bf16_t*)s->h2.p;
+  e2.h_out = h2_out ? h2_out : (bf16_t*)s->h2.p;
   e2.ld = s->Hp;
   e2.ldh = s->Kc;
   return mlp_gemm(ctx, s->h1.p, s->Hp, s->W2b.p, s->Hp, Bp, s->Hp, s->Hp, e2, st);
@@ -590,9 +598,11 @@ struct SurAcceptArgs {
   int C, D, Dz;
   float* z; float* r; float* z0; float* ghat; float* ghat0;
   double* logp; const double* K0; const double* ll;
+  float lik_weight;          // multiplies ll, as in the carried start-point log joint
   const float* ploc; const float* pscale;
   const double* u_fed; uint64_t seed, counter;
   float* sample_out; int32_t* accepted_out; int32_t* n_accept; int32_t* ndiv;
+  double* ratio_out;
   double* mom_mean; double* mom_m2; int64_t mom_n;
 };
 
@@ -615,7 +625,7 @@ __global__ void __launch_bounds__(256) sur_accept_kernel(SurAcceptArgs a) {
   plp = block_sum_f64<256>(plp, red);
   const double nbad = block_sum_f64<256>((double)bad, red);
   if (threadIdx.x == 0) {
-    const double logpL = a.ll[c] + plp;
+    const double logpL = (double)a.lik_weight * a.ll[c] + plp;
     const double ratio = a.K0[c] - 0.5 * kl + logpL - a.logp[c];
     const double u = a.u_fed ? a.u_fed[c] : accept_uniform(a.seed, a.counter, (uint32_t)c);
     const int acc = (log(u) < ratio) && nbad == 0.0;
@@ -624,6 +634,7 @@ __global__ void __launch_bounds__(256) sur_accept_kernel(SurAcceptArgs a) {
     if (a.accepted_out) a.accepted_out[c] = acc;
     if (a.n_accept) a.n_accept[c] += acc;
     if (a.ndiv && (nbad != 0.0 || !(ratio == ratio))) a.ndiv[c] += 1;
+    a.ratio_out[c] = ratio;
   }
   __syncthreads();
   const int acc = acc_flag;
@@ -648,8 +659,15 @@ __global__ void __launch_bounds__(256) sur_accept_kernel(SurAcceptArgs a) {
   }
 }
 
+// The [h2 | x] buffer of leapfrog step k (k & 1 selects one of the two).
+static bf16_t* xbuf(Surrogate* s, int k) {
+  return (bf16_t*)((k & 1) ? s->h2x.p : s->h2.p);
+}
+
+// Epilogue of step k's output layer; a middle step writes the next input
+// into step k + 1's buffer.
 static EpiLeapfrog lf_epi(edhmc_ctx* ctx, Surrogate* s, float half, float full, int last,
-                          int eval_only) {
+                          int eval_only, int k) {
   const float* P = (const float*)s->P.p;
   const float* nrm = (const float*)s->nrm.p;
   EpiLeapfrog e;
@@ -670,7 +688,7 @@ static EpiLeapfrog lf_epi(edhmc_ctx* ctx, Surrogate* s, float half, float full, 
   e.r = (float*)ctx->r.p;
   e.ghat = (float*)s->ghat.p;
   e.x0 = (bf16_t*)s->x0.p;
-  e.xcat = (bf16_t*)s->h2.p + s->Hp;
+  e.xcat = xbuf(s, k + 1) + s->Hp;
   e.xcat_ld = s->Kc;
   e.zlp = (bf16_t*)ctx->z_lp.p;
   e.zlp_plane = (int64_t)ctx->C_pad * ctx->D_pad;
@@ -704,7 +722,7 @@ static int eval_ghat(edhmc_ctx* ctx, Surrogate* s, hipStream_t st) {
                      (bf16_t*)s->h2.p + s->Hp, s->Kc);
   EDHMC_TRY(check_launch(ctx, "norm_in"));
   EDHMC_TRY(forward_hidden(ctx, s, Bp, false, st));
-  EpiLeapfrog e = lf_epi(ctx, s, 0.0f, 0.0f, 0, 1);
+  EpiLeapfrog e = lf_epi(ctx, s, 0.0f, 0.0f, 0, 1, 0);
   EDHMC_TRY(mlp_gemm(ctx, s->h2.p, s->Kc, s->W3b.p, s->Kc, Bp, s->Dp, s->Kc, e, st));
   s->ghat_valid = true;
   return EDHMC_OK;
@@ -733,7 +751,7 @@ static int sur_transition(edhmc_ctx* ctx, float eps, int L, const float* r0, con
   a.half = half; a.full = full;
   a.in_shift = nrm; a.in_scale = nrm + s->Dp;
   a.x0 = (bf16_t*)s->x0.p;
-  a.xcat = (bf16_t*)s->h2.p + s->Hp;
+  a.xcat = xbuf(s, 1) + s->Hp;
   a.xcat_ld = s->Kc;
   {
     ProfScope ps(ctx, PK_CHAIN, st);
@@ -742,9 +760,10 @@ static int sur_transition(edhmc_ctx* ctx, float eps, int L, const float* r0, con
   EDHMC_TRY(check_launch(ctx, "sur_begin"));
   const int Bp = (int)round_up(ctx->C, 128);
   for (int k = 1; k <= L; ++k) {
-    EDHMC_TRY(forward_hidden(ctx, s, Bp, false, st));
-    EpiLeapfrog e = lf_epi(ctx, s, half, full, k == L ? 1 : 0, 0);
-    EDHMC_TRY(mlp_gemm(ctx, s->h2.p, s->Kc, s->W3b.p, s->Kc, Bp, s->Dp, s->Kc, e, st));
+    bf16_t* xk = xbuf(s, k);
+    EDHMC_TRY(forward_hidden(ctx, s, Bp, false, st, xk));
+    EpiLeapfrog e = lf_epi(ctx, s, half, full, k == L ? 1 : 0, 0, k);
+    EDHMC_TRY(mlp_gemm(ctx, xk, s->Kc, s->W3b.p, s->Kc, Bp, s->Dp, s->Kc, e, st));
   }
   // exact log joint at the end point (z_lp planes written by the last step)
   EDHMC_TRY(glm_forward_ll(ctx, st));
@@ -754,10 +773,12 @@ static int sur_transition(edhmc_ctx* ctx, float eps, int L, const float* r0, con
   b.ghat = (float*)s->ghat.p; b.ghat0 = (float*)s->ghat0.p;
   b.logp = (double*)ctx->logp.p; b.K0 = (const double*)ctx->K0.p;
   b.ll = (const double*)ctx->ll.p;
+  b.lik_weight = ctx->lik_weight;
   b.ploc = (const float*)ctx->ploc.p; b.pscale = (const float*)ctx->pscale.p;
   b.u_fed = u; b.seed = seed; b.counter = counter;
   b.sample_out = sample_out; b.accepted_out = accepted_out; b.n_accept = n_accept;
   b.ndiv = (int32_t*)ctx->ndiv.p;
+  b.ratio_out = (double*)ctx->ratio.p;
   b.mom_mean = nullptr; b.mom_m2 = nullptr; b.mom_n = 0;
   if (ctx->mom_on) {
     ctx->mom_count += 1;
@@ -821,6 +842,7 @@ int edhmc_sur_create(edhmc_ctx* ctx, int32_t D, int32_t H, int32_t max_batch) {
   EDHMC_TRY(dev_alloc(ctx, s->h1, bh * 2));
   EDHMC_TRY(dev_alloc(ctx, s->a2, bh * 4));
   EDHMC_TRY(dev_alloc(ctx, s->h2, (size_t)s->Bp * s->Kc * 2));
+  EDHMC_TRY(dev_alloc(ctx, s->h2x, (size_t)s->Bp * s->Kc * 2));
   EDHMC_TRY(dev_alloc(ctx, s->dO, bd * 2));
   EDHMC_TRY(dev_alloc(ctx, s->da2, bh * 2));
   EDHMC_TRY(dev_alloc(ctx, s->da1, bh * 2));

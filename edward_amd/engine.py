@@ -344,6 +344,19 @@ class GLMEngine(Context):
                                         self.stream))
     return mean, sd
 
+  def last_ratio(self):
+    """FP64 log ratio K(r0) - K(rL) + log p(zL) - log p(z0) of every
+    chain's last transition (plain or surrogate), [C] device tensor."""
+    out = torch.empty(self.C, dtype=torch.float64, device=self.device)
+    self.check(self.lib.edhmc_last_ratio(self.h, L.ptr(out), self.stream))
+    return out
+
+  def last_momentum(self):
+    """Momenta at the end of every chain's last trajectory, [C, D]."""
+    out = torch.empty(self.C, self.D, device=self.device)
+    self.check(self.lib.edhmc_get_momentum(self.h, L.ptr(out), self.stream))
+    return out
+
   def draw_momenta(self, C, seed, counter):
     out = torch.empty(C, self.D, device=self.device)
     self.check(self.lib.edhmc_draw_momenta(self.h, C, self.D, int(seed), int(counter),

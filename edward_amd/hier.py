@@ -96,6 +96,12 @@ class HierEngine(Context):
     self.check(self.lib.edhmc_divergences(self.h, L.ptr(out), self.stream))
     return out
 
+  def last_ratio(self):
+    """FP64 Metropolis log ratio of every chain's last transition, [C]."""
+    out = torch.empty(self.C, dtype=torch.float64, device=self.device)
+    self.check(self.lib.edhmc_last_ratio(self.h, L.ptr(out), self.stream))
+    return out
+
 
 def sort_by_group(X, y, group, G=None):
   """Stable reorder of the rows by group; returns (X, y, offsets[G+1])."""

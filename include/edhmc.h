@@ -277,6 +277,18 @@ int edhmc_col_stats(edhmc_ctx* ctx, const float* X, int64_t P, int32_t D, float*
  * [C] int32 device array copy. */
 int edhmc_divergences(edhmc_ctx* ctx, int32_t* out, void* stream);
 
+/* Parity hook: the FP64 Metropolis log ratio
+ *   K(r0) - K(rL) + log p(zL) - log p(z0)
+ * of every chain's last edhmc_transition / edhmc_transition_surrogate
+ * (edward/inferences/hmc.py:100-107), [C] fp64 device array copy; the
+ * accept test was log(u) < ratio (hmc.py:108-109). */
+int edhmc_last_ratio(edhmc_ctx* ctx, double* ratio_out, void* stream);
+
+/* Parity hook: the momenta r_L at the end of every chain's last GLM
+ * transition (plain or surrogate; kept whether or not the proposal was
+ * accepted), r_out [C, D] fp32 -- the r of hmc.py:100-102's kinetic term. */
+int edhmc_get_momentum(edhmc_ctx* ctx, float* r_out, void* stream);
+
 /* Device RNG parity hooks: the momenta / uniforms edhmc_transition would
  * draw for (seed, counter).  r_out [C, D] fp32, u_out [C] fp64. */
 int edhmc_draw_momenta(edhmc_ctx* ctx, int32_t C, int32_t D, uint64_t seed,

@@ -34,6 +34,9 @@ Pinning (what each part is checked against, see tests/test_oracle.py):
                       inputs, eta = 0 and large-|eta| special cases,
                       scipy.stats log-pdfs.
   hier_logp_grad      finite differences, brute-force loops.
+  GLM.grad_bf16       the exact gradient where every residual is a bf16
+                      number (eta = 0), the bf16 bound elsewhere.
+  posterior_moments_is  quadrature of a skewed 2-D logistic posterior.
   hmc_transition      accept rule special cases (ratio >= 0, NaN, u = 0),
                       detailed balance on a 1-D target (exact posterior).
   run_hmc             index semantics (row 0 overwritten), n_accept.
@@ -161,6 +164,31 @@ class GLM(object):
   def loglik(self, W):
     """Likelihood part only (no prior), [C]."""
     return self.loglik_terms(self.eta(W)).sum(axis=1)
+
+  def grad_bf16(self, W, planes=2):
+    """The gradient with the roundings of the bf16 engine's gradient pass
+    (no reference counterpart; it restates the precision choice of the
+    device kernels so their trajectories can be followed closely): the
+    position enters eta as its bf16 hi plane (planes=1, the inner leapfrog
+    steps) or hi + lo planes (planes=2), the residuals y - sigmoid(eta) are
+    rounded to bf16 before X^T r, the prior term uses the fp32 position.
+    Products and sums in FP64."""
+    W = np.atleast_2d(np.asarray(W, np.float64))
+    w32 = W.astype(np.float32)
+    hi = _bf16(w32)
+    pos = hi if planes == 1 else hi + _bf16((w32 - hi.astype(np.float32)).astype(np.float32))
+    eta = pos @ self.X.T
+    res = _bf16(self.dloglik_deta(eta))
+    return res @ self.X + normal_logp_grad(w32.astype(np.float64), self.prior_loc,
+                                           self.prior_scale)
+
+
+def _bf16(a):
+  """Nearest bf16 number (ties to even) of the float32 value of a, as float64."""
+  a = np.ascontiguousarray(np.asarray(a, np.float64).astype(np.float32))
+  u = a.view(np.uint32).astype(np.uint64)
+  u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+  return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
 class GaussianTarget(object):
@@ -577,11 +605,13 @@ def min_ess(samples):
 
 
 def laplace_logreg(glm, w0=None, iters=50, tol=1e-10):
-  """MAP and inverse Hessian of a Bernoulli GLM by Newton's method (FP64).
+  """MAP and inverse Hessian of a Bernoulli GLM by Newton's method (FP64):
+  the Gaussian N(MAP, H^-1) of ``edward/inferences/laplace.py:17-137``.
 
-  At N = 1e6 rows the posterior mean of logistic regression differs from the
-  MAP by O(1/N) of its scale, far below Monte Carlo error, so
-  (MAP, H^-1) is the moment oracle for the large workloads.
+  Its moments differ from the posterior's by the posterior's skewness
+  (relative to the posterior scale, O(D / sqrt(N)) in the worst case), so the
+  moment oracle for the large workloads corrects them by importance sampling
+  (``posterior_moments_is``).
   """
   assert glm.likelihood == 'bernoulli'
   D = glm.D
@@ -600,3 +630,50 @@ def laplace_logreg(glm, w0=None, iters=50, tol=1e-10):
   p = sigmoid(eta)
   H = (glm.X * (p * (1 - p))[:, None]).T @ glm.X + np.diag(prec)
   return w, np.linalg.inv(H)
+
+
+def posterior_moments_is(glm, mode, cov, n_pairs, seed=0, chunk=256):
+  """Posterior mean and variance of a GLM, FP64, with standard errors.
+
+  Importance sampling from the Laplace approximation q = N(mode, cov) (its
+  Gaussian is what ``edward/inferences/laplace.py:17-137`` fits) with
+  antithetic pairs z = mode +- L xi, L L^T = cov, and self-normalised
+  weights p(z) / q(z).  The proposal's own moments are exact, so only the
+  weighted deviations from them are estimated (control variates): with
+  per-pair weights b_i = w_i+ + w_i- the mean is mode + sum_i (w_i+ - w_i-)
+  L xi_i / sum_i b_i, and E[(z - mode)^2] = diag(cov) + sum_i b_i ((L xi_i)^2
+  - diag(cov)) / sum_i b_i.  Near a Gaussian posterior w_i+ ~ w_i-, so the
+  first sum is small and its noise with it.  Standard errors are the ratio
+  estimator's delta-method ones over the n_pairs independent pairs.
+
+  No reference counterpart (Edward reports no posterior moments but the
+  sample store's, ``edward/models/empirical.py:87-96``); pinned against
+  quadrature of a skewed two-dimensional posterior (tests/test_oracle.py).
+  Returns dict(mean, var, se_mean, se_var, weight_ess).
+  """
+  mode = np.asarray(mode, np.float64)
+  D = mode.size
+  Lc = np.linalg.cholesky(np.asarray(cov, np.float64))
+  diag = np.einsum('ij,ij->i', Lc, Lc)
+  rng = np.random.default_rng(seed)
+  xi = rng.standard_normal((n_pairs, D))
+  dz = xi @ Lc.T
+  half_xi2 = 0.5 * (xi ** 2).sum(axis=1)
+  lw = np.empty((2, n_pairs))
+  for s in range(0, n_pairs, chunk):
+    e = min(n_pairs, s + chunk)
+    for k, sign in enumerate((1.0, -1.0)):
+      # log p(z) - log q(z) up to a constant
+      lw[k, s:e] = glm.logp(mode[None, :] + sign * dz[s:e]) + half_xi2[s:e]
+  lw -= lw.max()
+  w = np.exp(lw)
+  b = w[0] + w[1]
+  B = b.sum()
+  a = (w[0] - w[1])[:, None] * dz
+  delta = a.sum(axis=0) / B
+  c = b[:, None] * (dz ** 2 - diag[None, :])
+  e2 = c.sum(axis=0) / B
+  se_mean = np.sqrt(((a - delta[None, :] * b[:, None]) ** 2).sum(axis=0)) / B
+  se_var = np.sqrt(((c - e2[None, :] * b[:, None]) ** 2).sum(axis=0)) / B
+  return dict(mean=mode + delta, var=diag + e2 - delta ** 2, se_mean=se_mean, se_var=se_var,
+              weight_ess=float(B ** 2 / (w ** 2).sum()))

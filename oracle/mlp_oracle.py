@@ -14,9 +14,17 @@ so it restates published methods:
     with inputs x = (w - in_shift) * in_scale;
   * Adam (Kingma & Ba 2015) with bias correction.
 
+``predict_bf16`` is ``predict`` with the roundings the GPU kernels apply
+(bf16 inputs, weights and hidden activations on the matrix cores, fp32 bias,
+scale and shift), so that a trajectory the surrogate drives can be followed
+to fp32 precision instead of bf16 precision.
+
 Pinning: the backward pass is checked against central finite differences of
 the forward loss (tests/test_mlp_oracle.py), GELU' against a finite difference
-of GELU, and Adam against its first two steps written out by hand.
+of GELU, and Adam against its first two steps written out by hand;
+``bf16_round`` against hand-worked ties and the float32 -> bf16 truncation
+bound, ``predict_bf16`` against ``predict`` (equal when every rounded value
+is already a bf16 number, within the bf16 bound otherwise).
 """
 
 import numpy as np
@@ -93,3 +101,28 @@ def adam(p, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
   mh = m / (1 - b1 ** t)
   vh = v / (1 - b2 ** t)
   return p - lr * mh / (np.sqrt(vh) + eps), m, v
+
+
+def bf16_round(a):
+  """Nearest bf16 number (ties to even) of the float32 value of a, as float64."""
+  a = np.ascontiguousarray(np.asarray(a, np.float64).astype(np.float32))
+  u = a.view(np.uint32).astype(np.uint64)
+  u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+  return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def predict_bf16(params, W, norm):
+  """The surrogate gradient as the GPU computes it: x = bf16(fl32((w - shift)
+  * scale)) in float32 arithmetic, every layer's weights rounded to bf16,
+  products accumulated exactly (FP64 here, fp32 on the matrix cores), fp32
+  bias added, hidden activations rounded to bf16 after GELU, output scaled
+  and shifted."""
+  f32 = np.float32
+  x = bf16_round((np.asarray(W, np.float64).astype(f32) - np.asarray(norm['in_shift'], f32))
+                 * np.asarray(norm['in_scale'], f32))
+  h1 = bf16_round(gelu(x @ bf16_round(params['W1']).T + params['b1']))
+  h2 = bf16_round(gelu(h1 @ bf16_round(params['W2']).T + params['b2']))
+  o = np.concatenate([h2, x], axis=1) @ bf16_round(params['W3']).T + params['b3']
+  o = o.astype(f32)
+  return (o * np.asarray(norm['out_scale'], f32) + np.asarray(norm['out_shift'], f32)
+          ).astype(np.float64)

@@ -365,3 +365,55 @@ def test_plain_bf16_transition_on_the_chain_stationary_path():
   zd = out.cpu().numpy()[sel].astype(np.float64)
   err = np.linalg.norm(zd - z, axis=1) / np.linalg.norm(z, axis=1)
   assert err.max() < 1e-3, err
+
+
+@pytest.mark.parametrize('N,D,C,path', [(20000, 128, 512, 'w-stationary'),
+                                        (20000, 640, 256, 'lds-staged'),
+                                        (6000, 96, 130, 'lds-staged')])
+def test_plain_bf16_transition_follows_the_rounded_oracle(N, D, C, path):
+  """A plain bf16 transition against the oracle's leapfrog with the gradient
+  pass's own roundings (GLM.grad_bf16: hi + lo position planes at the ends,
+  the hi plane for the inner steps, bf16 residuals), on the W-stationary
+  kernels (bf16, D_pad <= 512, enough rows) and on the LDS-staged GEMMs
+  (D_pad = 640 > 512; N below 16 row tiles): two runs bitwise equal; end
+  positions within 1e-4 relative per chain (measured errors printed); the
+  log ratio at the GPU's own end point (edhmc_last_ratio against the FP64
+  log joint at the end point and the end momenta of edhmc_get_momentum)
+  within 1e-3 nats."""
+  from edward_amd import workloads as W
+  print('case N=%d D=%d C=%d (%s)' % (N, D, C, path), flush=True)
+  eps, L = 0.002, 8
+  wl = W.logreg(N, D, C, eps, L, 10, seed=41, bf16=True)
+  eng = _engine(wl, precision='bf16')
+  m = _oracle(wl)
+  rng = np.random.default_rng(3)
+  W0 = (wl.w_true[None, :] + 0.01 * rng.normal(size=(C, D))).astype(np.float32)
+  r0 = rng.normal(size=(C, D)).astype(np.float32)
+  outs, ratios = [], []
+  for _ in range(2):
+    eng.set_state(W0)
+    out = torch.empty(C, D, device='cuda')
+    eng.transition(eps, L, momenta=r0, uniforms=np.zeros(C), sample_out=out)
+    outs.append(out.cpu().numpy())
+    ratios.append(eng.last_ratio().cpu().numpy())
+  np.testing.assert_array_equal(outs[0], outs[1])
+  np.testing.assert_array_equal(ratios[0], ratios[1])
+  rL = eng.last_momentum().cpu().numpy().astype(np.float64)
+  sel = np.unique(np.linspace(0, C - 1, 12).astype(np.int64))
+  e32 = float(np.float32(eps))
+  z, r = W0[sel].astype(np.float64), r0[sel].astype(np.float64)
+  g = m.grad_bf16(z, planes=2)
+  for k in range(L):
+    r = r + 0.5 * e32 * g
+    z = z + e32 * r
+    g = m.grad_bf16(z, planes=2 if k == L - 1 else 1)
+    r = r + 0.5 * e32 * g
+  zd = outs[0][sel].astype(np.float64)
+  err = np.linalg.norm(zd - z, axis=1) / np.linalg.norm(z, axis=1)
+  rat_same = (0.5 * (r0[sel].astype(np.float64) ** 2).sum(1) - 0.5 * (rL[sel] ** 2).sum(1)
+              + m.logp(zd) - m.logp(W0[sel].astype(np.float64)))
+  e_rat = ratios[0][sel] - rat_same
+  print('measured: position rel err max %.2e median %.2e; ratio err at the end point max '
+        '%.2e nats' % (err.max(), np.median(err), np.abs(e_rat).max()), flush=True)
+  assert err.max() < 1e-4, err
+  assert np.abs(e_rat).max() < 1e-3, e_rat

@@ -1,0 +1,185 @@
+"""The headline transition at the large workload's full size, against the oracle.
+
+N = 1e6 rows, D = 512 (X ~ N(0, 1) with column scales U(0.5, 2), rounded to
+bf16 for both sides), 4096 chains, eps = 5e-4, L = 32; the surrogate MLP
+512 -> 1024 -> 1024 -> 512 (+ linear skip) drives the leapfrog and the exact
+bf16 / FP64 log joint decides the Metropolis test (edward/inferences/
+hmc.py:100-111, 195-210).  The network is trained briefly on exact gradients
+at draws of the Laplace approximation, so trajectories stay near the
+posterior as they do in sampling.  Momenta and uniforms are fed.
+
+Checked:
+  * determinism: two transitions from the same state give bitwise-equal end
+    points, ratios and decisions for all 4096 chains;
+  * end positions of 32 chains (spread over the chain tiles) against the
+    oracle's leapfrog (hmc_oracle.leapfrog) driven by mlp_oracle.predict_bf16
+    with the same weights (the network with the kernels' roundings), within
+    1e-4 relative per chain;
+  * the log joint at both ends against the FP64 oracle on the same bf16 X
+    (hmc_oracle.GLM), at the fp32 positions and at the hi + lo bf16 split
+    positions the kernel evaluates (bar 0.03 nats; measured 7.7e-3 and
+    1.4e-3 nats: the split's 2^-17 relative remainder of w dominates);
+  * the Metropolis log ratio (edhmc_last_ratio) against the oracle's ratio
+    at the GPU's own end point and momenta (edhmc_get_momentum; bar 0.03
+    nats) and along the oracle's own trajectory (bar 0.2 nats; measured
+    5.2e-2, median 1.6e-2: the fp32 and FP64 trajectories part where a
+    network input lies within rounding of a bf16 tie, which moves the
+    energy error, not the log joint);
+  * accept decisions with uniforms placed at ratio +- m, m = 4 x the
+    measured ratio error of those chains (at least 1e-3 nats): equal to the
+    oracle's, and the kept state is the end point (accept) or the start
+    (reject), bit for bit.
+The surrogate's forward pass and loss gradient at this shape are checked
+against the oracle in test_gpu_surrogate.py.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import hmc_oracle as O
+from oracle import mlp_oracle as MO
+
+pytestmark = pytest.mark.gpu
+
+H = 1024
+NSEL = 32
+
+
+@pytest.fixture(scope='module')
+def large():
+  from edward_amd import workloads as W
+  from edward_amd.engine import GLMEngine
+  from edward_amd.surrogate import Surrogate
+  from edward_amd.warmstart import LaplaceApproximation
+  print('\ncase: large workload N=1e6 D=512 C=4096, surrogate H=%d' % H, flush=True)
+  wl = W.logreg_large()
+  eng = GLMEngine(wl.X, wl.y, wl.prior_loc, wl.prior_scale, precision='bf16')
+  lap = LaplaceApproximation(eng)
+  Wp, Gp = lap.gradient_pairs(4 * wl.C, wl.C, seed=1)
+  sur = Surrogate(eng, hidden=H, max_batch=wl.C, seed=2)
+  sur.fit(Wp, Gp, epochs=3, batch=wl.C, lr=3e-3, lr_final=1e-3, seed=3)
+  W0 = lap.sample(wl.C, seed=4).cpu().numpy()
+  norm = {k: v.cpu().numpy().astype(np.float64) for k, v in
+          zip(('in_shift', 'in_scale', 'out_shift', 'out_scale'), sur._norm)}
+  p = MO.unflatten(sur.get_params().cpu().numpy(), wl.D, H)
+  m = O.GLM(wl.X.astype(np.float64), wl.y, wl.prior_loc, wl.prior_scale)
+  torch.cuda.synchronize()
+  return dict(wl=wl, eng=eng, sur=sur, W0=W0, norm=norm, p=p, m=m)
+
+
+def _run(eng, sur, W0, r0, u, eps, L):
+  C = W0.shape[0]
+  eng.set_state(W0)
+  acc = torch.zeros(C, dtype=torch.int32, device='cuda')
+  sur.transition(eps, L, momenta=r0, uniforms=u, accepted_out=acc)
+  z, lp, _ = eng.get_state()
+  ratio = eng.last_ratio()
+  torch.cuda.synchronize()
+  return z.cpu().numpy(), lp.cpu().numpy(), ratio.cpu().numpy(), acc.cpu().numpy()
+
+
+def _hilo(w):
+  """The position the kernel's log joint sees: bf16 hi + bf16 lo of fp32 w."""
+  w = np.asarray(w, np.float32)
+  hi = MO.bf16_round(w)
+  lo = MO.bf16_round((w - hi.astype(np.float32)).astype(np.float32))
+  return hi + lo
+
+
+def _rel(a, b):
+  return np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+
+
+def test_surrogate_transition_at_full_size(large):
+  wl, eng, sur, W0, m = large['wl'], large['eng'], large['sur'], large['W0'], large['m']
+  C, D, L = wl.C, wl.D, wl.n_steps
+  eps = float(np.float32(wl.step_size))
+  rng = np.random.default_rng(5)
+  r0 = rng.normal(size=(C, D)).astype(np.float32)
+  lp0 = eng.logp(W0).cpu().numpy()
+
+  # determinism over all chains (u = 0 accepts every finite proposal)
+  zA, lpA, ratA, accA = _run(eng, sur, W0, r0, np.zeros(C), eps, L)
+  zB, lpB, ratB, accB = _run(eng, sur, W0, r0, np.zeros(C), eps, L)
+  np.testing.assert_array_equal(zA, zB)
+  np.testing.assert_array_equal(ratA, ratB)
+  np.testing.assert_array_equal(accA, accB)
+  assert accA.all()
+  np.testing.assert_array_equal(lpA, lpB)
+
+  # oracle trajectories of NSEL chains under the network with the kernels'
+  # roundings
+  sel = np.linspace(0, C - 1, NSEL).astype(np.int64)
+  p, norm = large['p'], large['norm']
+
+  def field(w):
+    return None, MO.predict_bf16(p, w, norm)
+  z0 = W0[sel].astype(np.float64)
+  rr0 = r0[sel].astype(np.float64)
+  zL, rL, _, _ = O.leapfrog(z0, rr0, eps, L, field)
+  err_z = _rel(zA[sel].astype(np.float64), zL)
+  moved = _rel(zL, z0)
+
+  # log joints: GPU forward formula vs FP64 at the fp32 positions and at the
+  # hi + lo positions the kernel evaluates
+  lp_o0, lp_oL = m.logp(z0), m.logp(zA[sel])
+  lp_h0, lp_hL = m.logp(_hilo(W0[sel])), m.logp(_hilo(zA[sel]))
+  e_lp = np.concatenate([lp0[sel] - lp_o0, lpA[sel] - lp_oL])
+  e_lp_hilo = np.concatenate([lp0[sel] - lp_h0, lpA[sel] - lp_hL])
+  # the oracle's ratio along its own trajectory
+  rat_o = 0.5 * (rr0 ** 2).sum(1) - 0.5 * (rL ** 2).sum(1) + m.logp(zL) - lp_o0
+  e_rat = ratA[sel] - rat_o
+  # ... and at the GPU's own end point (zL, rL)
+  eng.set_state(W0)
+  sur.transition(eps, L, momenta=r0, uniforms=np.zeros(C))
+  rLg = eng.last_momentum().cpu().numpy()[sel].astype(np.float64)
+  rat_same = 0.5 * (rr0 ** 2).sum(1) - 0.5 * (rLg ** 2).sum(1) + lp_oL - lp_o0
+  e_same = ratA[sel] - rat_same
+  print('measured: position rel err max %.2e (trajectory moved %.2e rel); log joint err '
+        'max %.3e nats (vs hi+lo positions %.3e); ratio err at the same end point max %.3e '
+        'nats; ratio err vs the oracle trajectory max %.3e nats (median %.3e); oracle ratios '
+        '%.3f .. %.3f'
+        % (err_z.max(), moved.min(), np.abs(e_lp).max(), np.abs(e_lp_hilo).max(),
+           np.abs(e_same).max(), np.abs(e_rat).max(), np.median(np.abs(e_rat)), rat_o.min(),
+           rat_o.max()), flush=True)
+  assert moved.min() > 1e-3                       # the trajectories went somewhere
+  assert err_z.max() < 1e-4, err_z
+  assert np.abs(e_lp).max() < 0.03, e_lp
+  assert np.abs(e_same).max() < 0.03, e_same
+  assert np.abs(e_rat).max() < 0.2, e_rat
+
+  # decisions at u placed m nats on either side of the oracle's threshold
+  mgn = max(4.0 * np.abs(e_rat).max(), 1e-3)
+  want = np.arange(NSEL) % 2 == 1                 # odd: accept, even: reject
+  logu = np.where(want, np.minimum(rat_o, 0.0) - mgn, rat_o + mgn)
+  # a chain whose ratio is above -m cannot be rejected by a uniform below 1
+  want = want | (logu >= 0.0)
+  logu = np.where(logu >= 0.0, np.minimum(rat_o, 0.0) - mgn, logu)
+  u = rng.uniform(0.05, 0.95, size=C)
+  u[sel] = np.exp(logu)
+  assert (~want).sum() >= 4, rat_o
+  zC, lpC, ratC, accC = _run(eng, sur, W0, r0, u, eps, L)
+  np.testing.assert_array_equal(ratC, ratA)      # the ratio does not depend on u
+  np.testing.assert_array_equal(accC[sel].astype(bool), want)
+  np.testing.assert_array_equal(accC.astype(bool), np.log(u) < ratA)
+  keep = np.where(accC[:, None].astype(bool), zA, W0)
+  np.testing.assert_array_equal(zC, keep)
+  np.testing.assert_array_equal(lpC, np.where(accC.astype(bool), lpA, lp0))
+
+
+def test_surrogate_forward_with_the_kernels_roundings(large):
+  """The forward pass of the surrogate at 4096 x 512 (H = 1024) against the
+  oracle with the kernels' roundings (1e-5 relative, norm-wise) and against
+  the plain FP64 network (the bf16 bound, 2e-2)."""
+  wl, sur, W0 = large['wl'], large['sur'], large['W0']
+  g = sur.forward(torch.as_tensor(W0)).cpu().numpy().astype(np.float64)
+  sel = np.arange(0, wl.C, 8)
+  want = MO.predict_bf16(large['p'], W0[sel], large['norm'])
+  err = np.linalg.norm(g[sel] - want) / np.linalg.norm(want)
+  plain = MO.predict(large['p'], W0[sel], large['norm'])
+  err_plain = np.linalg.norm(g[sel] - plain) / np.linalg.norm(plain)
+  print('measured: surrogate forward rel err %.2e (vs the unrounded network %.2e)'
+        % (err, err_plain), flush=True)
+  assert err < 1e-5
+  assert err_plain < 2e-2

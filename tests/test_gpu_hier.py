@@ -229,3 +229,67 @@ def test_hier_many_groups_partial_last_tile():
   lp_ref, g_ref = m.logp_grad(th.astype(np.float64))
   assert np.all(np.abs(lp.cpu().numpy() - lp_ref) <= 1e-6 * np.abs(lp_ref) + 1e-3)
   assert np.abs(g.cpu().numpy() - g_ref).max() <= 1e-4 * np.abs(g_ref).max()
+
+
+def test_hier_full_size_logp_grad_and_transition():
+  """The hierarchical workload at its full size (workloads.hier_groups: G = 2048
+  groups of 1 + Poisson(400) rows, N ~ 821k, D = 64, 512 chains of 2114
+  coordinates, eps = 1e-3, L = 16): log joint and gradient of all chains on
+  the GPU, 4 of them against the oracle (bars of this file); one L = 16
+  transition with fed momenta: end positions of those chains within 1e-4
+  relative, the Metropolis log ratio (edhmc_last_ratio) against the
+  oracle's (measured error printed, bar 1e-2 nats: fp32 dynamics over 2114
+  coordinates move the kinetic energy by ~|r|^2 x 1e-6), and decisions with
+  uniforms placed at ratio +- m (m = 4 x the measured error, at least 1e-5
+  nats) equal to the oracle's."""
+  from edward_amd import workloads as W
+  wl = W.hier_groups()
+  print('\ncase: hier_groups G=%d N=%d D=%d C=%d' % (wl.G, wl.N, wl.D, wl.C), flush=True)
+  eng = _engine(wl.X, wl.y, wl.offsets)
+  m = O.HierLogReg(wl.X.astype(np.float64), wl.y, wl.offsets)
+  C, dim, eps, L = wl.C, wl.dim, wl.step_size, wl.n_steps
+  eps = float(np.float32(eps))
+  sel = np.array([0, 170, 341, 511])
+  th0 = wl.theta0
+  lp, g = eng.logp_grad(th0)
+  lp_ref, g_ref = m.logp_grad(th0[sel].astype(np.float64))
+  lps = lp.cpu().numpy()[sel]
+  gs = g.cpu().numpy()[sel].astype(np.float64)
+  print('measured: log joint err max %.3e nats (|logp| %.3e), gradient err max %.3e of '
+        'max|grad| %.3e' % (np.abs(lps - lp_ref).max(), np.abs(lp_ref).max(),
+                            np.abs(gs - g_ref).max(), np.abs(g_ref).max()), flush=True)
+  assert np.all(np.abs(lps - lp_ref) <= 1e-6 * np.abs(lp_ref) + 1e-3), (lps, lp_ref)
+  assert np.abs(gs - g_ref).max() <= 1e-4 * np.abs(g_ref).max()
+
+  rng = np.random.default_rng(51)
+  r0 = rng.normal(size=(C, dim)).astype(np.float32)
+  eng.set_state(th0)
+  acc = torch.zeros(C, dtype=torch.int32, device='cuda')
+  out_t = torch.empty(C, dim, device='cuda')
+  eng.transition(eps, L, momenta=r0, uniforms=np.zeros(C), accepted_out=acc, sample_out=out_t)
+  ratio = eng.last_ratio().cpu().numpy()
+  assert acc.cpu().numpy().all()
+  ref = O.hmc_transition(th0[sel].astype(np.float64), r0[sel].astype(np.float64),
+                         np.zeros(len(sel)), eps, L, m.logp_grad)
+  err_z = _rel_err(out_t.cpu().numpy()[sel].astype(np.float64), ref['zL'])
+  e_rat = ratio[sel] - ref['ratio']
+  print('measured: position rel err max %.2e; ratio err max %.3e nats (ratios %s)'
+        % (err_z.max(), np.abs(e_rat).max(), np.round(ref['ratio'], 4)), flush=True)
+  assert err_z.max() < 1e-4
+  assert np.abs(e_rat).max() < 1e-2
+
+  # decisions: even chains of sel rejected (where a uniform below 1 can do
+  # it), odd ones accepted
+  mgn = max(4.0 * np.abs(e_rat).max(), 1e-5)
+  rat = ref['ratio']
+  want = np.arange(len(sel)) % 2 == 1
+  logu = np.where(want, np.minimum(rat, 0.0) - mgn, rat + mgn)
+  want = want | (logu >= 0.0)
+  logu = np.where(logu >= 0.0, np.minimum(rat, 0.0) - mgn, logu)
+  u = rng.uniform(0.05, 0.95, size=C)
+  u[sel] = np.exp(logu)
+  eng.set_state(th0)
+  acc.zero_()
+  eng.transition(eps, L, momenta=r0, uniforms=u, accepted_out=acc)
+  np.testing.assert_array_equal(eng.last_ratio().cpu().numpy(), ratio)
+  np.testing.assert_array_equal(acc.cpu().numpy()[sel].astype(bool), want)

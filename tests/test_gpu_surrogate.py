@@ -47,8 +47,12 @@ def _rel(a, b):
   return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
 
 
-@pytest.mark.parametrize('D,H,B', [(24, 64, 300), (40, 200, 129), (24, 1000, 300)])
+@pytest.mark.parametrize('D,H,B', [(24, 64, 300), (40, 200, 129), (24, 1000, 300),
+                                   (512, 1024, 512)])
 def test_loss_grad_and_forward_match_oracle(D, H, B):
+  """The last case is the headline network (512 -> 1024 -> 1024 -> 512) on a
+  512-row batch."""
+  print('case D=%d H=%d B=%d' % (D, H, B), flush=True)
   _, eng = _engine(D=D)
   sur = _surrogate(eng, H=H, B=512, seed=1)
   rng = np.random.default_rng(2)
@@ -67,13 +71,14 @@ def test_loss_grad_and_forward_match_oracle(D, H, B):
   assert _rel(pred, MO.predict(p, W.astype(np.float64), norm)) < 2e-2
 
 
-def test_adam_step_matches_oracle_on_the_same_gradient():
-  _, eng = _engine(D=24)
-  sur = _surrogate(eng, H=64, B=256, seed=3)
+@pytest.mark.parametrize('D,H,B', [(24, 64, 256), (512, 1024, 512)])
+def test_adam_step_matches_oracle_on_the_same_gradient(D, H, B):
+  _, eng = _engine(D=D)
+  sur = _surrogate(eng, H=H, B=B, seed=3)
   rng = np.random.default_rng(4)
-  _set_norm(sur, _norm_np(rng, 24))
-  W = torch.tensor(rng.normal(size=(256, 24)).astype(np.float32), device='cuda')
-  G = torch.tensor(rng.normal(size=(256, 24)).astype(np.float32), device='cuda')
+  _set_norm(sur, _norm_np(rng, D))
+  W = torch.tensor(rng.normal(size=(B, D)).astype(np.float32), device='cuda')
+  G = torch.tensor(rng.normal(size=(B, D)).astype(np.float32), device='cuda')
   p0 = sur.get_params().cpu().numpy().astype(np.float64)
   _, grad = sur.loss_grad(W, G)
   sur.train_step(W, G, lr=1e-2)

@@ -71,3 +71,46 @@ def test_adam_first_steps():
   np.testing.assert_allclose(p1, p - 0.1 * np.sign(g), atol=1e-6)
   p2, m2, v2 = MO.adam(p1, g, m, v, 2, lr=0.1)
   np.testing.assert_allclose(p2, p1 - 0.1 * np.sign(g), atol=1e-6)
+
+
+def test_bf16_round_ties_to_even_and_bound():
+  one = 1.0
+  # 1 + 2^-8 lies halfway between 1 and 1 + 2^-7: ties to the even mantissa (1)
+  assert MO.bf16_round(one + 2.0 ** -8) == one
+  # 1 + 3 * 2^-8 lies halfway between 1 + 2^-7 (odd) and 1 + 2^-6 (even)
+  assert MO.bf16_round(one + 3 * 2.0 ** -8) == one + 2.0 ** -6
+  # just above a tie rounds up, just below rounds down
+  assert MO.bf16_round(one + 2.0 ** -8 + 2.0 ** -20) == one + 2.0 ** -7
+  assert MO.bf16_round(one + 2.0 ** -8 - 2.0 ** -20) == one
+  assert MO.bf16_round(-(one + 3 * 2.0 ** -8)) == -(one + 2.0 ** -6)
+  x = np.random.default_rng(3).normal(size=10000) * 10.0 ** np.random.default_rng(4).uniform(
+      -5, 5, size=10000)
+  r = MO.bf16_round(x)
+  bits = r.astype(np.float32).view(np.uint32)
+  assert np.all(bits & 0xFFFF == 0)                       # a bf16 number
+  # unit roundoff of an 8-bit significand: 2^-8
+  assert np.all(np.abs(r - x.astype(np.float32)) <= 2.0 ** -8 * np.abs(x) * (1 + 1e-6))
+  np.testing.assert_array_equal(MO.bf16_round(r), r)      # idempotent
+
+
+def test_predict_bf16_is_predict_up_to_the_roundings():
+  p, norm, W, _ = _setup(5, D=6, H=9, B=40)
+  # general case: within the bf16 bound of the FP64 predictor
+  a, b = MO.predict_bf16(p, W, norm), MO.predict(p, W, norm)
+  assert np.linalg.norm(a - b) / np.linalg.norm(b) < 2e-2
+  assert np.linalg.norm(a - b) / np.linalg.norm(b) > 1e-6   # the roundings do act
+  # every rounded value already a bf16 number: zero hidden layers (GELU(0) =
+  # 0), bf16 weights, inputs and normalisation giving bf16 inputs; only the
+  # fp32 output arithmetic remains
+  q = dict(p)
+  q['W1'] = np.zeros_like(p['W1'])
+  q['b1'] = np.zeros_like(p['b1'])
+  q['W2'] = np.zeros_like(p['W2'])
+  q['b2'] = np.zeros_like(p['b2'])
+  q['W3'] = MO.bf16_round(p['W3'])
+  nz = dict(norm)
+  nz['in_shift'] = np.zeros(6)
+  nz['in_scale'] = np.ones(6)
+  Wb = MO.bf16_round(W)
+  np.testing.assert_allclose(MO.predict_bf16(q, Wb, nz), MO.predict(q, Wb, nz), rtol=1e-6,
+                             atol=1e-6)

@@ -122,6 +122,29 @@ def test_logreg_special_cases():
     assert np.isfinite(m2.logp_grad(np.array([[-eta]]))[1]).all()
 
 
+def test_glm_grad_bf16_roundings():
+  """At W = 0 every residual y - 1/2 is a bf16 number and the prior
+  gradient vanishes: the rounded gradient is the exact one.  Elsewhere the
+  rounded residuals move each gradient entry by at most 2^-8 sum |x r|, and
+  the one-plane form differs from the two-plane form (its positions are
+  rounded)."""
+  rng = np.random.default_rng(9)
+  X = rng.normal(size=(300, 7))
+  y = (rng.random(300) < 0.4).astype(np.float64)
+  m = O.GLM(X, y, 0.0, 1.0)
+  W0 = np.zeros((1, 7))
+  np.testing.assert_array_equal(m.grad_bf16(W0), m.logp_grad(W0)[1])
+  W = rng.normal(size=(3, 7)) * 0.3
+  _, g = m.logp_grad(W)
+  res = y - O.sigmoid(W @ X.T)
+  bound = 2.0 ** -8 * (np.abs(res) @ np.abs(X)) + 1e-3 * np.abs(W).max() * np.abs(X).sum(0)
+  g2 = m.grad_bf16(W, planes=2)
+  assert np.all(np.abs(g2 - g) <= bound)
+  assert np.abs(g2 - g).max() > 0
+  g1 = m.grad_bf16(W, planes=1)
+  assert np.abs(g1 - g2).max() > 0
+
+
 def test_hier_gradient_matches_finite_differences_and_loops():
   rng = np.random.default_rng(5)
   G, D = 4, 3
@@ -459,3 +482,36 @@ def test_glm_neg_hessian_is_minus_the_gradient_jacobian(likelihood):
   np.testing.assert_allclose(P, -J, rtol=1e-6, atol=1e-6)
   np.testing.assert_allclose(P, P.T, atol=1e-12)
   assert np.linalg.eigvalsh(P).min() > 0
+
+
+def test_posterior_moments_is_matches_quadrature():
+  """A skewed two-dimensional logistic posterior (30 rows, large weights):
+  the importance-sampled moments agree with grid quadrature within 4
+  standard errors, while the Laplace approximation's own moments are off
+  by many (a dropped weight would not pass)."""
+  rng = np.random.default_rng(21)
+  N = 30
+  X = rng.normal(size=(N, 2)) * np.array([1.0, 2.0])
+  w_true = np.array([1.5, -1.0])
+  y = (rng.random(N) < O.sigmoid(X @ w_true)).astype(np.float64)
+  m = O.GLM(X, y, 0.0, 1.0)
+  mode, cov = O.laplace_logreg(m)
+  sd = np.sqrt(np.diag(cov))
+  # quadrature on a grid of +-9 Laplace sd
+  n = 601
+  g0 = np.linspace(mode[0] - 9 * sd[0], mode[0] + 9 * sd[0], n)
+  g1 = np.linspace(mode[1] - 9 * sd[1], mode[1] + 9 * sd[1], n)
+  G = np.stack(np.meshgrid(g0, g1, indexing='ij'), axis=-1).reshape(-1, 2)
+  lp = m.logp(G)
+  p = np.exp(lp - lp.max())
+  p /= p.sum()
+  q_mean = p @ G
+  q_var = p @ (G - q_mean) ** 2
+  est = O.posterior_moments_is(m, mode, cov, 20000, seed=3)
+  assert est['weight_ess'] > 0.5 * 40000
+  assert np.all(np.abs(est['mean'] - q_mean) < 4 * est['se_mean']), (est['mean'], q_mean)
+  assert np.all(np.abs(est['var'] - q_var) < 4 * est['se_var']), (est['var'], q_var)
+  assert np.all(est['se_mean'] < 0.02 * np.sqrt(q_var))
+  # the Laplace moments themselves are far outside those errors
+  assert np.any(np.abs(mode - q_mean) > 10 * est['se_mean'])
+  assert np.any(np.abs(np.diag(cov) - q_var) > 4 * est["se_var"])
