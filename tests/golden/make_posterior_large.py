@@ -9,7 +9,7 @@ from that Gaussian with antithetic pairs (posterior_moments_is), all on the
 bf16-rounded X in FP64.  The file also holds checksums of X and y, so a
 reader can tell that its data are the ones generated now.
 
-Run (about 5 minutes and 12 GB on 8 cores):
+Run (about 30 minutes and 12 GB on 8 cores):
     python tests/golden/make_posterior_large.py [n_pairs]
 """
 
@@ -37,7 +37,7 @@ def data_checksums(X, y):
 
 
 def main():
-  n_pairs = int(sys.argv[1]) if len(sys.argv) > 1 else 4096
+  n_pairs = int(sys.argv[1]) if len(sys.argv) > 1 else 65536
   t0 = time.time()
   wl = workloads.logreg_large()
   print('workload %.1f s' % (time.time() - t0), flush=True)

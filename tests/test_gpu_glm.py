@@ -376,10 +376,16 @@ def test_plain_bf16_transition_follows_the_rounded_oracle(N, D, C, path):
   the hi plane for the inner steps, bf16 residuals), on the W-stationary
   kernels (bf16, D_pad <= 512, enough rows) and on the LDS-staged GEMMs
   (D_pad = 640 > 512; N below 16 row tiles): two runs bitwise equal; end
-  positions within 1e-4 relative per chain (measured errors printed); the
-  log ratio at the GPU's own end point (edhmc_last_ratio against the FP64
-  log joint at the end point and the end momenta of edhmc_get_momentum)
-  within 1e-3 nats."""
+  positions of every chain: median within 1e-5 relative (measured 1e-7 to
+  7e-7), every chain within 1e-3 (measured up to 1.6e-4 on isolated chains
+  of either kernel path: where a coordinate lies within fp32 rounding of a
+  bf16 tie the two sides' hi planes differ by one bf16 ulp, which
+  test_oracle.py::test_bf16_inner_positions_amplify_rounding_differences
+  reproduces with the oracle alone); the log ratio at the GPU's own end
+  point (edhmc_last_ratio against the FP64 log joint at the end point and
+  the end momenta of edhmc_get_momentum) within 5e-3 nats (measured up to
+  1.2e-3 at D = 640: the hi + lo split's 2^-17 remainder of w times the
+  gradient)."""
   from edward_amd import workloads as W
   print('case N=%d D=%d C=%d (%s)' % (N, D, C, path), flush=True)
   eps, L = 0.002, 8
@@ -399,7 +405,7 @@ def test_plain_bf16_transition_follows_the_rounded_oracle(N, D, C, path):
   np.testing.assert_array_equal(outs[0], outs[1])
   np.testing.assert_array_equal(ratios[0], ratios[1])
   rL = eng.last_momentum().cpu().numpy().astype(np.float64)
-  sel = np.unique(np.linspace(0, C - 1, 12).astype(np.int64))
+  sel = np.arange(C)
   e32 = float(np.float32(eps))
   z, r = W0[sel].astype(np.float64), r0[sel].astype(np.float64)
   g = m.grad_bf16(z, planes=2)
@@ -413,7 +419,9 @@ def test_plain_bf16_transition_follows_the_rounded_oracle(N, D, C, path):
   rat_same = (0.5 * (r0[sel].astype(np.float64) ** 2).sum(1) - 0.5 * (rL[sel] ** 2).sum(1)
               + m.logp(zd) - m.logp(W0[sel].astype(np.float64)))
   e_rat = ratios[0][sel] - rat_same
-  print('measured: position rel err max %.2e median %.2e; ratio err at the end point max '
-        '%.2e nats' % (err.max(), np.median(err), np.abs(e_rat).max()), flush=True)
-  assert err.max() < 1e-4, err
-  assert np.abs(e_rat).max() < 1e-3, e_rat
+  print('measured: position rel err max %.2e median %.2e (%d chains above 1e-5); ratio err '
+        'at the end point max %.2e nats' % (err.max(), np.median(err), (err > 1e-5).sum(),
+                                            np.abs(e_rat).max()), flush=True)
+  assert np.median(err) < 1e-5, err
+  assert err.max() < 1e-3, err
+  assert np.abs(e_rat).max() < 5e-3, e_rat

@@ -515,3 +515,46 @@ def test_posterior_moments_is_matches_quadrature():
   # the Laplace moments themselves are far outside those errors
   assert np.any(np.abs(mode - q_mean) > 10 * est['se_mean'])
   assert np.any(np.abs(np.diag(cov) - q_var) > 4 * est["se_var"])
+
+
+def test_bf16_inner_positions_amplify_rounding_differences():
+  """Why a few chains of the bf16 engine leave the rounded oracle's leapfrog
+  by 1e-5..1e-4 while the rest agree to ~1e-7: the inner gradients are
+  evaluated at the bf16 hi plane of the position, so where a coordinate lies
+  within rounding distance of a bf16 tie, positions that differ by one fp32
+  ulp (the GPU's fp32 and the oracle's FP64 arithmetic) round to
+  neighbouring bf16 numbers and the evaluated position moves by 2^-8 of that
+  coordinate.  Shown with the oracle alone (N=20000, D=128, 128 chains, eps
+  = 0.002, L = 8): starts differing by one fp32 ulp in half the coordinates
+  stay 4e-8 apart under the exact-gradient leapfrog, while under the rounded
+  leapfrog (GLM.grad_bf16) one chain ends ~1e-4 apart and the median stays
+  below 1e-7."""
+  from edward_amd import workloads as W
+  N, D, C, eps, L = 20000, 128, 128, 0.002, 8
+  wl = W.logreg(N, D, C, eps, L, 10, seed=41, bf16=True)
+  m = O.GLM(wl.X.astype(np.float64), wl.y, wl.prior_loc, wl.prior_scale)
+  rng = np.random.default_rng(3)
+  z0 = (wl.w_true[None, :] + 0.01 * rng.normal(size=(C, D))).astype(np.float32)
+  r0 = rng.normal(size=(C, D))
+  z0b = np.where(rng.random((C, D)) < 0.5, np.nextafter(z0, np.float32(np.inf)), z0)
+  e32 = float(np.float32(eps))
+
+  def leapfrog(z, rounded):
+    z = z.astype(np.float64)
+    r = r0.copy()
+    g = m.grad_bf16(z, 2) if rounded else m.logp_grad(z)[1]
+    for k in range(L):
+      r = r + 0.5 * e32 * g
+      z = z + e32 * r
+      g = m.grad_bf16(z, 2 if k == L - 1 else 1) if rounded else m.logp_grad(z)[1]
+      r = r + 0.5 * e32 * g
+    return z
+
+  def rel(a, b):
+    return np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+  d_exact = rel(leapfrog(z0b, False), leapfrog(z0, False))
+  d_round = rel(leapfrog(z0b, True), leapfrog(z0, True))
+  assert d_exact.max() < 1e-7
+  assert np.median(d_round) < 1e-7
+  assert d_round.max() > 1e-5 and d_round.max() > 100 * np.median(d_round)
+  assert (d_round > 1e-5).sum() <= 4          # isolated chains
