@@ -70,6 +70,7 @@ SIGNATURES = {
     'edhmc_record_pairs': (ctypes.c_int, [_c_p, _c_p, _c_p, _i64, _i32]),
     'edhmc_record_count': (_i64, [_c_p]),
     'edhmc_col_stats': (ctypes.c_int, [_c_p, _c_p, _i64, _i32, _c_p, _c_p, _c_p]),
+    'edhmc_col_stats_noalloc': (ctypes.c_int, [_c_p, _c_p, _i64, _i32, _c_p, _c_p, _c_p]),
     'edhmc_profile_enable': (ctypes.c_int, [_c_p, _i32]),
     'edhmc_profile_read': (ctypes.c_int, [_c_p, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(_i64)]),

@@ -13,6 +13,7 @@
 // Xc [D, N] (GEMM2's B), so every MFMA operand is K-contiguous.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "common.h"
 #include "ctx.h"
@@ -1710,6 +1711,46 @@ static int do_transition(edhmc_ctx* ctx, float eps, int L, const float* r0, cons
 
 using namespace edhmc;
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry points: one file-scope device array,
+// one copy per GPU (held until the process ends).  Host code never uses the
+// array's name as an address: its device address on each GPU comes from
+// hipGetSymbolAddress, taken once per device at edhmc_create with that device
+// current, and only that address is handed to kernels.  Calls that overlap
+// on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+constexpr int kStatsChunks = 256;      // row chunks of the column statistics
+constexpr int kNoAllocMaxD = 4096;     // widest X of edhmc_col_stats_noalloc
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[(size_t)kStatsChunks * kNoAllocMaxD * 2];
+static std::mutex g_scratch_mu;
+static double* g_scratch[kMaxDevices] = {};
+
+static int scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return EDHMC_EINVAL;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return EDHMC_OK;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+    (void)hipGetLastError();
+    return EDHMC_EHIP;
+  }
+  g_scratch[device] = (double*)p;
+  return EDHMC_OK;
+}
+
+// Column sums and sums of squares per row chunk into part [chunks][D][2]
+// (FP64), then mean and standard deviation (divisor P) in a fixed order.
+static int col_stats_launch(edhmc_ctx* ctx, const float* X, int64_t P, int D, int chunks,
+                            double* part, float* mean_out, float* std_out, hipStream_t st) {
+  hipLaunchKernelGGL(col_stats_part_kernel, dim3((D + 255) / 256, chunks), dim3(256), 0, st, X, P,
+                     D, chunks, part);
+  EDHMC_TRY(check_launch(ctx, "col_stats"));
+  hipLaunchKernelGGL(col_stats_final_kernel, dim3((D + 255) / 256), dim3(256), 0, st,
+                     (const double*)part, P, D, chunks, mean_out, std_out);
+  return check_launch(ctx, "col_stats_final");
+}
+
 edhmc_ctx::~edhmc_ctx() {
   for (hipEvent_t e : prof.pool) (void)hipEventDestroy(e);
   if (sur) surrogate_destroy(sur);
@@ -1731,6 +1772,8 @@ int edhmc_create(int device, edhmc_ctx** out) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return EDHMC_EHIP;
   if (device < 0 || device >= n) return EDHMC_EINVAL;
+  DeviceGuard dg(device);
+  if (scratch_init(device) != EDHMC_OK) return EDHMC_EHIP;
   edhmc_ctx* c = new (std::nothrow) edhmc_ctx();
   if (!c) return EDHMC_ENOMEM;
   c->device = device;
@@ -2349,15 +2392,28 @@ int edhmc_col_stats(edhmc_ctx* ctx, const float* X, int64_t P, int32_t D, float*
   DeviceGuard dg(ctx->device);
   if (!X || !mean_out || !std_out || P <= 0 || D <= 0)
     return set_err(ctx, EDHMC_EINVAL, "bad arguments");
-  hipStream_t st = (hipStream_t)stream;
-  const int chunks = (int)std::min<int64_t>(256, P);
+  const int chunks = (int)std::min<int64_t>(kStatsChunks, P);
   EDHMC_TRY(dev_alloc(ctx, ctx->stats_part, (size_t)chunks * D * 16, false));
-  hipLaunchKernelGGL(col_stats_part_kernel, dim3((D + 255) / 256, chunks), dim3(256), 0, st, X, P,
-                     D, chunks, (double*)ctx->stats_part.p);
-  EDHMC_TRY(check_launch(ctx, "col_stats"));
-  hipLaunchKernelGGL(col_stats_final_kernel, dim3((D + 255) / 256), dim3(256), 0, st,
-                     (const double*)ctx->stats_part.p, P, D, chunks, mean_out, std_out);
-  return check_launch(ctx, "col_stats_final");
+  return col_stats_launch(ctx, X, P, D, chunks, (double*)ctx->stats_part.p, mean_out, std_out,
+                          (hipStream_t)stream);
+}
+
+int edhmc_col_stats_noalloc(edhmc_ctx* ctx, const float* X, int64_t P, int32_t D,
+                            float* mean_out, float* std_out, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  if (!X || !mean_out || !std_out || P <= 0 || D <= 0)
+    return set_err(ctx, EDHMC_EINVAL, "bad arguments");
+  if (D > kNoAllocMaxD)
+    return set_err(ctx, EDHMC_EINVAL,
+                   "D=%d exceeds the %d columns the allocation-free scratch holds", D,
+                   kNoAllocMaxD);
+  double* part = g_scratch[ctx->device];
+  if (!part) return set_err(ctx, EDHMC_ESTATE, "no scratch address for device %d", ctx->device);
+  // every (chunk, column) slot the final pass reads is written by the first
+  // pass of this call: nothing needs clearing
+  const int chunks = (int)std::min<int64_t>(kStatsChunks, P);
+  return col_stats_launch(ctx, X, P, D, chunks, part, mean_out, std_out, (hipStream_t)stream);
 }
 
 int edhmc_autocov(edhmc_ctx* ctx, const float* s, int64_t T, int32_t C, int32_t D, int32_t max_lag,

@@ -273,6 +273,17 @@ int edhmc_sample_moments(edhmc_ctx* ctx, const void* params, int32_t params_dtyp
 int edhmc_col_stats(edhmc_ctx* ctx, const float* X, int64_t P, int32_t D, float* mean_out,
                     float* std_out, void* stream);
 
+/* The same column statistics, allocating nothing: no counterpart in the
+ * reference -- for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The partial sums live in a 16 MiB file-scope device array of the library
+ * (one per GPU, held until the process ends) instead of context scratch;
+ * D is limited to 4096 (EDHMC_EINVAL beyond).  Results are bitwise those of
+ * edhmc_col_stats.  Calls that overlap on one GPU share the array and race:
+ * serialise them (one stream, or an external lock). */
+int edhmc_col_stats_noalloc(edhmc_ctx* ctx, const float* X, int64_t P, int32_t D,
+                            float* mean_out, float* std_out, void* stream);
+
 /* Number of chains whose proposal was non-finite (rejected) since binding,
  * [C] int32 device array copy. */
 int edhmc_divergences(edhmc_ctx* ctx, int32_t* out, void* stream);
