@@ -64,6 +64,7 @@ SIGNATURES = {
     'edhmc_moments_get': (ctypes.c_int, [_c_p, _c_p, _c_p, ctypes.POINTER(_i64), _c_p]),
     'edhmc_divergences': (ctypes.c_int, [_c_p, _c_p, _c_p]),
     'edhmc_last_ratio': (ctypes.c_int, [_c_p, _c_p, _c_p]),
+    'edhmc_set_inner_gradient': (ctypes.c_int, [_c_p, _i32]),
     'edhmc_get_momentum': (ctypes.c_int, [_c_p, _c_p, _c_p]),
     'edhmc_sample_moments': (ctypes.c_int, [_c_p, _c_p, _i32, _i64, _i64, _i64, _c_p, _c_p,
                                             _c_p]),
@@ -87,6 +88,10 @@ SIGNATURES = {
     'edhmc_sur_loss_grad': (ctypes.c_int, [_c_p, _c_p, _c_p, _i32, _c_p, _c_p, _c_p]),
     'edhmc_sur_train_step': (ctypes.c_int, [_c_p, _c_p, _c_p, _i32, _f32, _f32, _f32, _f32,
                                             _i32, _c_p, _c_p]),
+    'edhmc_gather_rows': (ctypes.c_int, [_c_p, _c_p, _i64, _i32, _c_p, _i32, _c_p, _c_p]),
+    'edhmc_sur_skip_gram': (ctypes.c_int, [_c_p, _c_p, _c_p, _i64, _c_p, _i32, _c_p, _c_p,
+                                           _c_p]),
+    'edhmc_affine_rows': (ctypes.c_int, [_c_p, _c_p, _i64, _i32, _c_p, _c_p, _c_p, _c_p]),
     'edhmc_sur_train_epoch': (ctypes.c_int, [_c_p, _c_p, _c_p, _i64, _c_p, _i32, _i32,
                                              ctypes.POINTER(_f32), _f32, _f32, _f32, _i32, _c_p,
                                              _c_p]),

@@ -81,6 +81,7 @@ struct edhmc_ctx {
   int C = 0, C_pad = 0;
   bool state_valid = false;
   bool g_valid = false;   // g holds the exact gradient at z
+  bool inner_exact = false;   // bf16: inner leapfrog gradients on both position planes
   edhmc::DevBuf z, r, g, z0, g0, z_lp, logp, K0, flag, ndiv;
   edhmc::DevBuf ratio;   // [C] FP64 Metropolis log ratio of the last HMC transition
   // ---- gradient scratch ----

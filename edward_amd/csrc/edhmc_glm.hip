@@ -1094,10 +1094,11 @@ static bool ws_grad_ok(const edhmc_ctx* ctx) {
 static int grad_eval(edhmc_ctx* ctx, bool split, bool ll, hipStream_t st) {
   if (ctx->cdt == EDHMC_BF16) {
     if (split) {
-      if (ll && ws_grad_ok(ctx)) {
+      if (ws_grad_ok(ctx)) {
         // the W-stationary kernel writes the residuals beside its
-        // log-likelihood partials (the forward-only pass's formula)
-        EDHMC_TRY(launch_ws_forward(ctx, st, true));
+        // log-likelihood partials (the forward-only pass's formula; reduced
+        // only when asked for)
+        EDHMC_TRY(launch_ws_forward(ctx, st, true, 2, ll));
       } else if (ll) {
         EDHMC_TRY((launch_gemm1<bf16_t, CfgP2, 2, true, true>(ctx, st)));
       } else {
@@ -1477,7 +1478,7 @@ static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st, bool wr, int planes
                          ctx->N, ctx->N_pad, rows_per_part, parts, mt_per_xcd,
                          part_per_xcd ? 1 : 0, (double*)ctx->llpart.p, (int64_t)ctx->C_pad,
                          (const float*)ctx->y.p, (bf16_t*)ctx->Rt.p, (int64_t)ctx->C_pad);
-    else if (wr && planes == 2 && ll)
+    else if (wr && planes == 2)
       hipLaunchKernelGGL((gemm_ll_ws_kernel<true, 2, true, kWsPreGrad>), dim3(grid),
                          dim3(kWsThreads), 0, st, (const bf16_t*)ctx->z_lp.p,
                          (int64_t)ctx->C_pad * ctx->D_pad, ctx->D_pad, (const bf16_t*)ctx->Xr.p,
@@ -1676,7 +1677,9 @@ static int do_transition(edhmc_ctx* ctx, float eps, int L, const float* r0, cons
   EDHMC_TRY(check_launch(ctx, "chain_begin"));
   for (int k = 1; k <= L; ++k) {
     const bool last = k == L;
-    EDHMC_TRY(grad_eval(ctx, last, last, st));
+    // bf16: the inner gradients see the hi plane of the positions unless
+    // exact inner gradients were asked for (edhmc_set_inner_gradient)
+    EDHMC_TRY(grad_eval(ctx, last || ctx->inner_exact, last, st));
     if (!last) {
       a.rec_W = a.rec_G = nullptr;
       if (ctx->rec_W && k % ctx->rec_stride == 0 && ctx->rec_count + ctx->C <= ctx->rec_cap) {
@@ -2317,6 +2320,13 @@ int edhmc_divergences(edhmc_ctx* ctx, int32_t* out, void* stream) {
   if (hipMemcpyAsync(out, ctx->ndiv.p, (size_t)ctx->C * 4, hipMemcpyDeviceToDevice,
                      (hipStream_t)stream) != hipSuccess)
     return set_err(ctx, EDHMC_EHIP, "copy failed");
+  return EDHMC_OK;
+}
+
+int edhmc_set_inner_gradient(edhmc_ctx* ctx, int32_t exact) {
+  if (!ctx) return EDHMC_EINVAL;
+  if (exact != 0 && exact != 1) return set_err(ctx, EDHMC_EINVAL, "exact must be 0 or 1");
+  ctx->inner_exact = exact != 0;
   return EDHMC_OK;
 }
 

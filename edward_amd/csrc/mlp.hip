@@ -314,6 +314,110 @@ __global__ void rowsum_bf16_kernel(const bf16_t* __restrict__ in, int R, int K, 
   if (lane == 0) out[row] = s;
 }
 
+// Least-squares Gram products of the output layer's linear skip over the
+// rows idx[0..n) of (W, G): x = (w - in_shift) * in_scale and t = (g -
+// out_shift) / out_scale in fp32, out = sum_rows x^T x (blockIdx.z = 0) or
+// x^T t (z = 1), [D x D] FP64.  A 64 x 64 output block per workgroup, a
+// thread owning 4 x 4 entries, rows staged 32 at a time through LDS.  An
+// index outside [0, P) contributes NaN.
+__global__ void __launch_bounds__(256)
+skip_gram_kernel(const float* __restrict__ W, const float* __restrict__ G, int64_t P, int D,
+                 const int64_t* __restrict__ idx, int n, const float* __restrict__ nrm, int Dp,
+                 double* __restrict__ xtx, double* __restrict__ xtt) {
+  __shared__ float la[32][64], lb[32][64];
+  const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64, z = blockIdx.z;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const float* in_shift = nrm;
+  const float* in_scale = nrm + Dp;
+  const float* out_shift = nrm + 2 * Dp;
+  const float* out_scale = nrm + 3 * Dp;
+  double acc[4][4] = {};
+  for (int r0 = 0; r0 < n; r0 += 32) {
+    for (int q = threadIdx.x; q < 32 * 64; q += 256) {
+      const int rr = q >> 6, c = q & 63, row = r0 + rr;
+      float a = 0.0f, b = 0.0f;
+      if (row < n && (idx[row] < 0 || idx[row] >= P)) {
+        a = b = __builtin_nanf("");
+      } else if (row < n) {
+        const int64_t src = idx[row] * (int64_t)D;
+        if (i0 + c < D) a = (W[src + i0 + c] - in_shift[i0 + c]) * in_scale[i0 + c];
+        const int j = j0 + c;
+        if (j < D) b = z == 0 ? (W[src + j] - in_shift[j]) * in_scale[j]
+                              : (G[src + j] - out_shift[j]) / out_scale[j];
+      }
+      la[rr][c] = a;
+      lb[rr][c] = b;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int rr = 0; rr < 32; ++rr) {
+      float av[4], bv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        av[u] = la[rr][ty * 4 + u];
+        bv[u] = lb[rr][tx * 4 + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] += (double)av[u] * (double)bv[v];
+    }
+    __syncthreads();
+  }
+  double* out = z == 0 ? xtx : xtt;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = i0 + ty * 4 + u, j = j0 + tx * 4 + v;
+      if (i < D && j < D) out[(int64_t)i * D + j] = acc[u][v];
+    }
+}
+
+// out [n x D] = shift + E [n x D] R [D x D] (fp32; 64 x 64 output blocks, a
+// thread owning 4 x 4 entries, K staged 32 at a time through LDS).
+__global__ void __launch_bounds__(256)
+affine_rows_kernel(const float* __restrict__ E, int64_t n, int D, const float* __restrict__ R,
+                   const float* __restrict__ shift, float* __restrict__ out) {
+  __shared__ float le[64][33], lr[32][64];
+  const int64_t r0 = (int64_t)blockIdx.x * 64;
+  const int c0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  float acc[4][4] = {};
+  for (int k0 = 0; k0 < D; k0 += 32) {
+    for (int q = threadIdx.x; q < 64 * 32; q += 256) {
+      const int rr = q >> 5, kk = q & 31;
+      const int64_t row = r0 + rr;
+      le[rr][kk] = (row < n && k0 + kk < D) ? E[row * D + k0 + kk] : 0.0f;
+      const int k2 = q >> 6, cc = q & 63;
+      lr[k2][cc] = (k0 + k2 < D && c0 + cc < D) ? R[(int64_t)(k0 + k2) * D + c0 + cc] : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int kk = 0; kk < 32; ++kk) {
+      float ev[4], rv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        ev[u] = le[ty * 4 + u][kk];
+        rv[u] = lr[kk][tx * 4 + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = fmaf(ev[u], rv[v], acc[u][v]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t row = r0 + ty * 4 + u;
+      const int c = c0 + tx * 4 + v;
+      if (row < n && c < D) out[row * D + c] = shift[c] + acc[u][v];
+    }
+}
+
 // Adam (Kingma & Ba 2015) over the padded parameter vector.
 __global__ void adam_kernel(float* __restrict__ P, const float* __restrict__ G, float* __restrict__ M,
                             float* __restrict__ V, int64_t n, float lr, float b1, float b2,
@@ -1011,6 +1115,46 @@ int edhmc_sur_train_epoch(edhmc_ctx* ctx, const float* W, const float* G, int64_
                                    losses_out + b, stream));
   }
   return EDHMC_OK;
+}
+
+int edhmc_gather_rows(edhmc_ctx* ctx, const float* src, int64_t P, int32_t D,
+                      const int64_t* idx, int32_t B, float* dst, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  if (!src || !idx || !dst || P <= 0 || D <= 0 || B < 0)
+    return set_err(ctx, EDHMC_EINVAL, "bad arguments");
+  if (B == 0) return EDHMC_OK;
+  hipLaunchKernelGGL(gather_rows_kernel, dim3(blocks_for((int64_t)B * D)), dim3(256), 0,
+                     (hipStream_t)stream, src, P, D, idx, B, dst);
+  return check_launch(ctx, "gather_rows");
+}
+
+int edhmc_sur_skip_gram(edhmc_ctx* ctx, const float* W, const float* G, int64_t P,
+                        const int64_t* idx, int32_t n, double* xtx_out, double* xtt_out,
+                        void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  EDHMC_TRY(check_sur(ctx));
+  Surrogate* s = ctx->sur;
+  if (!s->norm_set) return set_err(ctx, EDHMC_ESTATE, "normalisation not set");
+  if (!W || !G || !idx || !xtx_out || !xtt_out || n <= 0 || P <= 0)
+    return set_err(ctx, EDHMC_EINVAL, "bad arguments");
+  const int nb = (s->D + 63) / 64;
+  hipLaunchKernelGGL(skip_gram_kernel, dim3(nb, nb, 2), dim3(256), 0, (hipStream_t)stream, W, G,
+                     P, s->D, idx, n, (const float*)s->nrm.p, s->Dp, xtx_out, xtt_out);
+  return check_launch(ctx, "skip_gram");
+}
+
+int edhmc_affine_rows(edhmc_ctx* ctx, const float* E, int64_t n, int32_t D, const float* R,
+                      const float* shift, float* out, void* stream) {
+  if (!ctx) return EDHMC_EINVAL;
+  DeviceGuard dg(ctx->device);
+  if (!E || !R || !shift || !out || n < 0 || D <= 0)
+    return set_err(ctx, EDHMC_EINVAL, "bad arguments");
+  if (n == 0) return EDHMC_OK;
+  hipLaunchKernelGGL(affine_rows_kernel, dim3((unsigned)((n + 63) / 64), (D + 63) / 64),
+                     dim3(256), 0, (hipStream_t)stream, E, n, D, R, shift, out);
+  return check_launch(ctx, "affine_rows");
 }
 
 int edhmc_transition_surrogate(edhmc_ctx* ctx, float eps, int32_t n_steps, const float* r0,

@@ -1,7 +1,9 @@
 """Device engines: thin owners of an edhmc context.
 
-torch is used only to hold device memory and to name the stream; all
-arithmetic of the sampler runs in libedhmc.so's kernels.
+torch holds device memory, names the stream and copies buffers; the
+arithmetic of the sampler, the warm start and the surrogate's training runs
+in libedhmc.so's kernels (small D-sized steps -- a least-squares solve, a
+Cholesky factor -- on the host in FP64).
 """
 
 import ctypes
@@ -190,6 +192,11 @@ class GLMEngine(Context):
       w, lp, g, P = wn, lpn, gn, Pn
     return w, P, lp, it
 
+  def set_inner_gradient(self, exact):
+    """bf16: inner leapfrog gradients on both position planes (exact=True)
+    or on the bf16-rounded positions (False, the default)."""
+    self.check(self.lib.edhmc_set_inner_gradient(self.h, 1 if exact else 0))
+
   def set_lik_weight(self, w):
     """Multiplier of the likelihood in the log joint (Edward's ``scale`` of
     the observed variable, used by the stochastic-gradient samplers)."""
@@ -355,6 +362,23 @@ class GLMEngine(Context):
     """Momenta at the end of every chain's last trajectory, [C, D]."""
     out = torch.empty(self.C, self.D, device=self.device)
     self.check(self.lib.edhmc_get_momentum(self.h, L.ptr(out), self.stream))
+    return out
+
+  def gather_rows(self, src, idx):
+    """src [P, D] rows idx (device int64) -> a new [len(idx), D] tensor."""
+    src = src.contiguous()
+    P, D = src.shape
+    out = torch.empty(int(idx.numel()), D, device=self.device)
+    self.check(self.lib.edhmc_gather_rows(self.h, L.ptr(src), int(P), int(D), L.ptr(idx),
+                                          int(idx.numel()), L.ptr(out), self.stream))
+    return out
+
+  def affine_rows(self, E, R, shift):
+    """shift + E R for E [n, D], R [D, D], shift [D] (device fp32)."""
+    n, D = E.shape
+    out = torch.empty(n, D, device=self.device)
+    self.check(self.lib.edhmc_affine_rows(self.h, L.ptr(E), int(n), int(D), L.ptr(R),
+                                          L.ptr(shift), L.ptr(out), self.stream))
     return out
 
   def draw_momenta(self, C, seed, counter):

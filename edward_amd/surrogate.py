@@ -66,19 +66,23 @@ class Surrogate(object):
     o['b3'] = o['W3'] + D * (H + D)
     return o
 
-  def init_skip_least_squares(self, W, G, max_rows=32768, ridge=1e-6):
+  def init_skip_least_squares(self, W, G, rows=None, max_rows=32768, ridge=1e-6):
     """Start the output layer's linear skip at the least-squares map from the
-    normalised inputs to the normalised targets (on a row sample: the Gram
-    products on the device in fp32, the D x D solve on the host in FP64)."""
+    normalised inputs to the normalised targets, over an evenly spaced sample
+    of ``rows`` (default all) of the pairs (W, G): the Gram products in FP64
+    on the device (``edhmc_sur_skip_gram``), the D x D solve on the host."""
+    eng = self.engine
     P = int(W.shape[0])
-    idx = np.linspace(0, P - 1, min(P, max_rows)).astype(np.int64)
-    sel = torch.as_tensor(idx, device=self.engine.device)
-    ins, insc, outs, outsc = [v.to(self.engine.device, torch.float32) for v in self._norm]
-    x = (W.index_select(0, sel) - ins) * insc
-    t = (G.index_select(0, sel) - outs) / outsc
-    xtx = (x.t() @ x).double().cpu().numpy()
-    xtt = (x.t() @ t).double().cpu().numpy()
-    A = np.linalg.solve(xtx + ridge * len(idx) * np.eye(self.D), xtt)   # t ~ x A
+    rows = np.arange(P) if rows is None else np.asarray(rows, np.int64)
+    sel = rows[np.linspace(0, len(rows) - 1, min(len(rows), max_rows)).astype(np.int64)]
+    idx = torch.as_tensor(sel, dtype=torch.int64).to(eng.device)
+    xtx = torch.empty(self.D, self.D, dtype=torch.float64, device=eng.device)
+    xtt = torch.empty_like(xtx)
+    eng.check(eng.lib.edhmc_sur_skip_gram(eng.h, L.ptr(W), L.ptr(G), P, L.ptr(idx), len(sel),
+                                          L.ptr(xtx), L.ptr(xtt), eng.stream))
+    xtx = xtx.cpu().numpy()
+    xtt = xtt.cpu().numpy()
+    A = np.linalg.solve(xtx + ridge * len(sel) * np.eye(self.D), xtt)   # t ~ x A
     flat = self.get_params().cpu().numpy()
     o = self._offsets()
     W3 = flat[o['W3']:o['b3']].reshape(self.D, self.H + self.D)
@@ -110,11 +114,12 @@ class Surrogate(object):
         self.engine.h, *[L.ptr(v) for v in vs], self.engine.stream))
 
   def normalise_from(self, W, G):
-    """Per-coordinate standardisation of inputs and targets from the data."""
-    wm, ws = self.engine.col_stats(W)
-    gm, gs = self.engine.col_stats(G)
-    ws = torch.where(ws > 0, ws, torch.ones_like(ws))
-    gs = torch.where(gs > 0, gs, torch.ones_like(gs))
+    """Per-coordinate standardisation of inputs and targets from the data
+    (column statistics on the device, the D-vector arithmetic on the host)."""
+    wm, ws = [v.cpu().numpy().astype(np.float64) for v in self.engine.col_stats(W)]
+    gm, gs = [v.cpu().numpy().astype(np.float64) for v in self.engine.col_stats(G)]
+    ws = np.where(ws > 0, ws, 1.0)
+    gs = np.where(gs > 0, gs, 1.0)
     self.set_normalisation(wm, 1.0 / ws, gm, gs)
 
   # -- compute -------------------------------------------------------------
@@ -175,40 +180,44 @@ class Surrogate(object):
 
   def fit(self, W, G, epochs=20, batch=4096, lr=1e-3, lr_final=1e-4, seed=0, normalise=True,
           log=None, val_frac=0.0):
-    """Adam on the pairs (W [P, D], G [P, D], device tensors), shuffled every
-    epoch, learning rate decayed geometrically from lr to lr_final.  With
-    val_frac > 0 that fraction of the pairs is held out and the weights of
-    the epoch with the lowest held-out loss are kept (early stopping).
+    """Adam on the pairs (W [P, D], G [P, D], device tensors), reshuffled
+    every epoch, learning rate decayed geometrically from lr to lr_final.
+    With val_frac > 0 that fraction of the pairs is held out and the weights
+    of the epoch with the lowest held-out loss are kept (early stopping).
+    Shuffles are host permutations of row indices; the batches are gathered
+    on the device (``edhmc_sur_train_epoch``, ``edhmc_gather_rows``).
     Returns the mean training loss of each epoch."""
-    W = W.to(self.engine.device, torch.float32).contiguous()
-    G = G.to(self.engine.device, torch.float32).contiguous()
+    eng = self.engine
+    dev = eng.device
+    W = W.to(dev, torch.float32).contiguous()
+    G = G.to(dev, torch.float32).contiguous()
+    P_all = int(W.shape[0])
+    rng = np.random.default_rng(int(seed))
+    order = rng.permutation(P_all)
+    nv = max(1, int(val_frac * P_all)) if val_frac > 0 else 0
+    train_rows = order[nv:]
     Wv = Gv = None
-    if val_frac > 0:
-      gen0 = torch.Generator(device=self.engine.device)
-      gen0.manual_seed(int(seed) + 1)
-      perm0 = torch.randperm(int(W.shape[0]), generator=gen0, device=self.engine.device)
-      nv = max(1, int(val_frac * int(W.shape[0])))
-      Wv, Gv = W.index_select(0, perm0[:nv]), G.index_select(0, perm0[:nv])
-      W, G = W.index_select(0, perm0[nv:]), G.index_select(0, perm0[nv:])
-    P = int(W.shape[0])
+    if nv:
+      vi = torch.as_tensor(order[:nv], dtype=torch.int64).to(dev)
+      Wv = eng.gather_rows(W, vi)
+      Gv = eng.gather_rows(G, vi)
+    P = len(train_rows)
     batch = min(int(batch), self.max_batch, P)
     if normalise:
       self.normalise_from(W, G)
-      self.init_skip_least_squares(W, G)
+      self.init_skip_least_squares(W, G, rows=train_rows)
     best = None
     if Wv is not None:
       best = (self.validation_loss(Wv, Gv), self.get_params().clone())
-    gen = torch.Generator(device=self.engine.device)
-    gen.manual_seed(int(seed))
     n_batches = P // batch
     total = max(1, epochs * n_batches)
     decay = (lr_final / lr) ** (1.0 / total) if lr_final else 1.0
     losses = []
     step = 0
     for ep in range(epochs):
-      perm = torch.randperm(P, generator=gen, device=self.engine.device)
+      perm = torch.as_tensor(train_rows[rng.permutation(P)], dtype=torch.int64).to(dev)
       lrs = [lr * decay ** (step + b) for b in range(n_batches)]
-      ep_loss = self.train_epoch(W, G, perm, batch, lrs).mean().item()
+      ep_loss = float(self.train_epoch(W, G, perm, batch, lrs).cpu().numpy().mean())
       step += n_batches
       losses.append(ep_loss)
       msg = 'surrogate epoch %d loss %.5f' % (ep, ep_loss)

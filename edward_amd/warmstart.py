@@ -30,11 +30,11 @@ class LaplaceApproximation(object):
     self._root = torch.as_tensor(root, dtype=torch.float32, device=dev)
 
   def sample(self, n, seed=0):
-    """n draws [n, D] (float32, on the engine's device)."""
-    g = torch.Generator(device=self._mode.device)
-    g.manual_seed(int(seed))
-    e = torch.randn(int(n), self._mode.numel(), generator=g, device=self._mode.device)
-    return self._mode + e @ self._root
+    """n draws [n, D] (float32, on the engine's device): standard normal rows
+    from the library's Philox stream keyed by ``seed``, mapped by
+    mode + e L^-1 on the device (``edhmc_affine_rows``)."""
+    e = self.engine.draw_momenta(int(n), int(seed), 0)
+    return self.engine.affine_rows(e, self._root, self._mode)
 
   def gradient_pairs(self, n, batch, seed=0):
     """(W, G): n draws and the exact gradient of the log joint at each,

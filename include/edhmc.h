@@ -288,6 +288,15 @@ int edhmc_col_stats_noalloc(edhmc_ctx* ctx, const float* X, int64_t P, int32_t D
  * [C] int32 device array copy. */
 int edhmc_divergences(edhmc_ctx* ctx, int32_t* out, void* stream);
 
+/* bf16 models: exact = 1 makes the inner leapfrog gradients of
+ * edhmc_transition use both bf16 planes of the positions (hi + lo, the
+ * positions to 2^-17, as the first and last gradients always do); the
+ * default 0 uses the hi plane only (the positions rounded to bf16, about a
+ * third cheaper).  Either way the accept test of hmc.py:100-111 uses the
+ * exact log joint, so this changes the proposal, not the target.  fp32
+ * models ignore it. */
+int edhmc_set_inner_gradient(edhmc_ctx* ctx, int32_t exact);
+
 /* Parity hook: the FP64 Metropolis log ratio
  *   K(r0) - K(rL) + log p(zL) - log p(z0)
  * of every chain's last edhmc_transition / edhmc_transition_surrogate
@@ -361,6 +370,27 @@ int edhmc_sur_train_epoch(edhmc_ctx* ctx, const float* W, const float* G, int64_
                           const int64_t* perm, int32_t batch, int32_t n_batches,
                           const float* lr_host, float beta1, float beta2, float adam_eps,
                           int32_t step0, double* losses_out, void* stream);
+
+/* Rows of a device matrix: dst [B, D] = src [P, D] rows idx[0..B) (int64);
+ * an index outside [0, P) gives a NaN row.  (The surrogate's held-out
+ * training pairs; no reference counterpart.) */
+int edhmc_gather_rows(edhmc_ctx* ctx, const float* src, int64_t P, int32_t D,
+                      const int64_t* idx, int32_t B, float* dst, void* stream);
+
+/* Gram products for the least-squares start of the surrogate's linear skip:
+ * over the rows idx[0..n) of the pairs (W [P, D], G [P, D]), with the
+ * normalisation set by edhmc_sur_set_norm, x = (w - in_shift) * in_scale and
+ * t = (g - out_shift) / out_scale, xtx_out = sum x^T x, xtt_out = sum x^T t,
+ * [D, D] fp64.  No reference counterpart. */
+int edhmc_sur_skip_gram(edhmc_ctx* ctx, const float* W, const float* G, int64_t P,
+                        const int64_t* idx, int32_t n, double* xtx_out, double* xtt_out,
+                        void* stream);
+
+/* out [n, D] = shift [D] + E [n, D] R [D, D] (fp32): draws of a Gaussian
+ * N(shift, R^T R) from standard normal rows E (the Laplace warm start's
+ * draws, edward/inferences/laplace.py:17-137 fits that Gaussian). */
+int edhmc_affine_rows(edhmc_ctx* ctx, const float* E, int64_t n, int32_t D, const float* R,
+                      const float* shift, float* out, void* stream);
 /* HMC transition whose leapfrog forces come from the surrogate; the ratio
  * uses the exact log joint at the end point (bf16 X with a hi/lo bf16 split
  * of w, FP64 reduction), so the chain targets the exact posterior.  Same

@@ -697,3 +697,32 @@ def posterior_moments_is(glm, mode, cov, n_pairs, seed=0, chunk=256):
   se_var = np.sqrt(((c - e2[None, :] * b[:, None]) ** 2).sum(axis=0)) / B
   return dict(mean=mode + delta, var=diag + e2 - delta ** 2, se_mean=se_mean, se_var=se_var,
               weight_ess=float(B ** 2 / (w ** 2).sum()))
+
+
+def logreg_grad_f32(X, y, W, threads=None):
+  """One plain fp32 gradient evaluation of the logistic log joint for C
+  chains, ``hmc.py:199``'s ``tf.gradients`` of ``_log_joint`` in closed
+  form (prior N(0, I)): eta = W X^T, r = y - sigmoid(eta), grad = r X - W,
+  every step in float32 -- the CPU restatement the benchmark times.  The
+  GEMMs run in the host BLAS (all its threads); the elementwise sigmoid runs
+  over column blocks in ``threads`` Python threads (numpy releases the GIL).
+  X [N, D] float32, y [N] float32, W [C, D] float32."""
+  X = np.asarray(X, np.float32)
+  y = np.asarray(y, np.float32)
+  W = np.asarray(W, np.float32)
+  eta = W @ X.T
+  n = eta.shape[1]
+  nt = threads or (os.cpu_count() or 1)
+  edges = np.linspace(0, n, 4 * nt + 1).astype(np.int64)
+
+  def block(k):
+    e = eta[:, edges[k]:edges[k + 1]]
+    np.negative(e, out=e)
+    with np.errstate(over='ignore'):     # exp(-eta) = inf gives sigmoid 0
+      np.exp(e, out=e)
+    e += np.float32(1.0)
+    np.reciprocal(e, out=e)
+    np.subtract(y[None, edges[k]:edges[k + 1]], e, out=e)
+  with ThreadPoolExecutor(nt) as ex:
+    list(ex.map(block, range(len(edges) - 1)))
+  return eta @ X - W

@@ -425,3 +425,41 @@ def test_plain_bf16_transition_follows_the_rounded_oracle(N, D, C, path):
   assert np.median(err) < 1e-5, err
   assert err.max() < 1e-3, err
   assert np.abs(e_rat).max() < 5e-3, e_rat
+
+
+@pytest.mark.parametrize('N,D,C', [(20000, 128, 512), (20000, 640, 256)])
+def test_plain_bf16_transition_with_exact_inner_gradients(N, D, C):
+  """edhmc_set_inner_gradient(1): every leapfrog gradient on both position
+  planes (W-stationary and LDS-staged kernels): end positions of every chain
+  against the oracle's leapfrog with GLM.grad_bf16(planes=2) at every step,
+  median within 1e-5 and every chain within 1e-4 relative (measured median
+  1.0e-6 / 1.5e-6, max 2.8e-6 / 3.1e-6: no hi-plane rounding flips remain,
+  bf16 residual flips do)."""
+  from edward_amd import workloads as W
+  print('case N=%d D=%d C=%d' % (N, D, C), flush=True)
+  eps, L = 0.002, 8
+  wl = W.logreg(N, D, C, eps, L, 10, seed=43, bf16=True)
+  eng = _engine(wl, precision='bf16')
+  eng.set_inner_gradient(True)
+  m = _oracle(wl)
+  rng = np.random.default_rng(4)
+  W0 = (wl.w_true[None, :] + 0.01 * rng.normal(size=(C, D))).astype(np.float32)
+  r0 = rng.normal(size=(C, D)).astype(np.float32)
+  eng.set_state(W0)
+  out = torch.empty(C, D, device='cuda')
+  eng.transition(eps, L, momenta=r0, uniforms=np.zeros(C), sample_out=out)
+  e32 = float(np.float32(eps))
+  z, r = W0.astype(np.float64), r0.astype(np.float64)
+  g = m.grad_bf16(z, planes=2)
+  for _ in range(L):
+    r = r + 0.5 * e32 * g
+    z = z + e32 * r
+    g = m.grad_bf16(z, planes=2)
+    r = r + 0.5 * e32 * g
+  zd = out.cpu().numpy().astype(np.float64)
+  err = np.linalg.norm(zd - z, axis=1) / np.linalg.norm(z, axis=1)
+  print('measured: position rel err max %.2e median %.2e' % (err.max(), np.median(err)),
+        flush=True)
+  assert np.median(err) < 1e-5, err
+  assert err.max() < 1e-4, err
+  eng.set_inner_gradient(False)

@@ -267,7 +267,7 @@ def test_transition_ratio_and_selection():
   out = O.hmc_transition(W, r0, u, 0.05, 5, m.logp_grad)
   zL, rL, lpL, _ = O.leapfrog(W, r0, 0.05, 5, m.logp_grad)
   ratio = 0.5 * (r0 ** 2).sum(1) - 0.5 * (rL ** 2).sum(1) + lpL - m.logp(W)
-  np.testing.assert_allclose(out['ratio'], ratio, rtol=1e-13)
+  np.testing.assert_allclose(out['ratio'], ratio, rtol=1e-13, atol=1e-12)
   for c in range(5):
     acc = math.log(u[c]) < ratio[c] if u[c] > 0 else True
     assert out['accept'][c] == acc
@@ -558,3 +558,16 @@ def test_bf16_inner_positions_amplify_rounding_differences():
   assert np.median(d_round) < 1e-7
   assert d_round.max() > 1e-5 and d_round.max() > 100 * np.median(d_round)
   assert (d_round > 1e-5).sum() <= 4          # isolated chains
+
+
+def test_logreg_grad_f32_matches_the_fp64_gradient():
+  """The fp32 CPU restatement the benchmark times equals the FP64 gradient to
+  fp32 accuracy (threads only split the elementwise part)."""
+  rng = np.random.default_rng(0)
+  X = rng.normal(size=(3000, 16)).astype(np.float32)
+  y = (rng.random(3000) < 0.5).astype(np.float32)
+  W = (rng.normal(size=(5, 16)) * 0.3).astype(np.float32)
+  g = O.logreg_grad_f32(X, y, W, threads=3)
+  _, ref = O.GLM(X.astype(np.float64), y, 0.0, 1.0).logp_grad(W.astype(np.float64))
+  assert g.dtype == np.float32
+  assert np.abs(g - ref).max() < 1e-5 * np.abs(ref).max()
