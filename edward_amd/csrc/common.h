@@ -89,26 +89,19 @@ __device__ inline float log1p_unit(float e) {
   return u == 1.0f ? e : l * (e * fast_rcp(u - 1.0f));
 }
 
-// Negative Bernoulli log-likelihood softplus(eta) - y eta for y in [0, 1] as a
-// sum of non-negative terms,
-//   log1p(exp(-|eta|)) + y max(-eta, 0) + (1 - y) max(eta, 0),
-// so nothing cancels (for y in {0, 1} one of the max terms is selected
-// exactly).  Summed over the rows, a rounding error d of eta then enters
-// through (sigmoid(eta) - y) d, whose weights average to zero over the data
-// at the posterior -- not through sigmoid(eta) d > 0, as it would if the
-// linear part w . X^T y were added exactly beside a sum of softplus(eta).
-// log1p(t) for t = 2^(-|eta| log2 e) in (0, 1] is ln2 log2(1 + t), or
-// t (1 - t/2) once 1 + t would round t away (t < 2^-12: the dropped t^3/3 is
-// below 2^-36); absolute error about 2^-24 per element.  Both forms are
-// evaluated and one is selected: a branch per element (what the compiler
-// makes of a conditional around the logarithm) costs an exec mask
-// save/restore pair in the middle of the epilogues.
+// Negative Bernoulli log-likelihood softplus(eta) - y eta for y in [0, 1]:
+//   log1p(exp(-|eta|)) + (max(eta, 0) - y eta),
+// log1p(t) for t = 2^(-|eta| log2 e) as ln2 log2(1 + t) (absolute error about
+// 2^-24: 1 + t rounds; once it rounds to 1 the dropped t is below 2^-24 too),
+// the second term one fma (exact for y in {0, 1}: 0 or |eta|).  Every term
+// uses the same rounded eta, so summed over the rows a rounding error d of
+// eta enters through (sigmoid(eta) - y) d, whose weights average to zero over
+// the data at the posterior -- not through sigmoid(eta) d > 0, as it would if
+// the linear part w . X^T y were added exactly beside a sum of softplus(eta).
 __device__ inline float bern_nll_fast(float eta, float y) {
   const float t = __builtin_amdgcn_exp2f(-fabsf(eta) * 1.4426950408889634f);
-  const float big = __builtin_amdgcn_logf(1.0f + t) * 0.6931471805599453f;
-  const float small = t * (1.0f - 0.5f * t);
-  const float l = t < 2.44140625e-4f ? small : big;
-  return l + y * fmaxf(-eta, 0.0f) + (1.0f - y) * fmaxf(eta, 0.0f);
+  return fmaf(__builtin_amdgcn_logf(1.0f + t), 0.6931471805599453f,
+              fmaf(-y, eta, fmaxf(eta, 0.0f)));
 }
 
 __device__ inline float glm_dll(const LikParams& p, float eta, float y) {

@@ -1273,6 +1273,12 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = f32x4v_t{0.0f, 0.0f, 0.0f, 0.0f};
+    // forward: y of the wave's rows in the tile computed now (tile 0 first)
+    float y_cur = 0.0f;
+    if (!WR) {
+      const int64_t n = n_begin + wave * kWsWaveRows + lane;
+      y_cur = y[n < N_pad ? n : N_pad - 1];
+    }
     // (t, ks0): the block of PRE k steps computed now; (tn, kn): the
     // block whose loads are issued meanwhile (the last block again past the
     // end: in bounds, never used)
@@ -1288,13 +1294,16 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
       const char* rp[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) rp[j] = row_ptr(tl, j) + kl * 64;
-      // forward: y of the wave's 64 rows of tile t, lane l holding row l (one
-      // coalesced load per k block, issued before this block's X loads so
-      // that the epilogue waits for it alone; padded rows read y = 0)
-      float yw = 0.0f;
+      // forward: y of the wave's 64 rows of the next block's tile (lane l
+      // holding row l, one coalesced load per k block; padded rows read
+      // y = 0).  It is carried into the next iteration (y_cur below), whose
+      // epilogue uses it: a load consumed in its own iteration's epilogue is
+      // placed inside that branch and waited for together with every X
+      // prefetch in flight.
+      float y_next = 0.0f;
       if (!WR) {
-        const int64_t n = n_begin + t * kWsTile + wave * kWsWaveRows + lane;
-        yw = y[n < N_pad ? n : N_pad - 1];
+        const int64_t n = n_begin + tl * kWsTile + wave * kWsWaveRows + lane;
+        y_next = y[n < N_pad ? n : N_pad - 1];
       }
 #pragma unroll
       for (int u = 0; u < PRE; ++u) {
@@ -1326,7 +1335,7 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             valid[j] = nb + 16 * j < n_end;
-            yv[j] = __shfl(yw, 16 * j + l16, 64);
+            yv[j] = __shfl(y_cur, 16 * j + l16, 64);
           }
 #pragma unroll
           for (int i = 0; i < 4; ++i)
@@ -1371,11 +1380,9 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
                 // softplus and sigmoid from one t = exp(-|eta|)
                 const float tt = __builtin_amdgcn_exp2f(-fabsf(eta) * 1.4426950408889634f);
                 if (LL) {
-                  // softplus(eta) - y eta as non-negative terms (bern_nll_fast)
-                  const float big = __builtin_amdgcn_logf(1.0f + tt) * 0.6931471805599453f;
-                  const float small = tt * (1.0f - 0.5f * tt);
-                  const float nl = (tt < 2.44140625e-4f ? small : big) +
-                                   yv[v] * fmaxf(-eta, 0.0f) + (1.0f - yv[v]) * fmaxf(eta, 0.0f);
+                  // softplus(eta) - y eta as in bern_nll_fast
+                  const float nl = fmaf(__builtin_amdgcn_logf(1.0f + tt), 0.6931471805599453f,
+                                        fmaf(-yv[v], eta, fmaxf(eta, 0.0f)));
                   s32 -= v < left ? nl : 0.0f;
                 }
                 const float inv = fast_rcp(1.0f + tt);
@@ -1397,6 +1404,7 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
       }
       t = tn;
       ks0 = kn;
+      if (!WR) y_cur = y_next;
     }
   }
   if (!LL) return;   // uniform across the workgroup: no barrier below is skipped by part of it
