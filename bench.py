@@ -4,7 +4,7 @@ surrogate-driven bf16 HMC versus plain fp32 HMC.
 Workload (default): N = 1e6 rows, D = 512 features (X ~ N(0,1) with column
 scales U(0.5, 2), stored bf16), 4096 chains from N(0, I), eps = 5e-4,
 L = 32 leapfrog steps; the surrogate MLP 512 -> 1024 -> 1024 -> 512 (+ linear
-skip) trained on ~200k (w, grad log p) pairs from the plain sampler's own
+skip) trained on ~254k (w, grad log p) pairs from the plain sampler's own
 warm-up trajectories -- the large logistic-regression configuration with the
 learned-surrogate recipe.  A step is one HMC transition of all chains.
 
@@ -71,7 +71,7 @@ def parse():
                   help='untimed plain bf16 transitions (kernel load, first touch)')
   ap.add_argument('--burn-in', type=int, default=80,
                   help='max plain bf16 descent transitions before pair recording starts')
-  ap.add_argument('--burn-in-steps', type=int, default=8,
+  ap.add_argument('--burn-in-steps', type=int, default=4,
                   help='leapfrog steps of the descent transitions')
   ap.add_argument('--rec-start-tol', type=float, default=100.0,
                   help='pair recording starts when the mean log joint rises less than this '
@@ -98,10 +98,10 @@ def parse():
                        "of HBM traffic profiles; 'forward': --steps exact forward passes "
                        "(the gemm_ll kernel alone) on the large workload, for profiling")
   ap.add_argument('--hidden', type=int, default=1024)
-  ap.add_argument('--epochs', type=int, default=32)
+  ap.add_argument('--epochs', type=int, default=16)
   ap.add_argument('--lr', type=float, default=3e-3)
-  ap.add_argument('--rec-stride', type=int, default=2)
-  ap.add_argument('--rec-transitions', type=int, default=4)
+  ap.add_argument('--rec-stride', type=int, default=1)
+  ap.add_argument('--rec-transitions', type=int, default=2)
   ap.add_argument('--surrogate-only', action='store_true',
                   help='stop after the surrogate phase (tuning runs)')
   ap.add_argument('--seed', type=int, default=0)
@@ -238,7 +238,9 @@ def laplace_variant(bf, wl, args, K):
   torch.cuda.synchronize()
   warm_s = time.time() - t0
   sur = Surrogate(bf, hidden=args.hidden, max_batch=C, seed=args.seed)
-  sur.fit(Wp, Gp, epochs=args.epochs, batch=C, lr=args.lr, lr_final=5e-5, seed=args.seed,
+  # 32 epochs over its 114,688 pairs: about as many Adam steps as the
+  # recipe's 16 epochs over ~254k
+  sur.fit(Wp, Gp, epochs=32, batch=C, lr=args.lr, lr_final=5e-5, seed=args.seed,
           val_frac=0.05)
   torch.cuda.synchronize()
   train_s = time.time() - t0 - warm_s
