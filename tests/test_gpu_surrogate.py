@@ -287,3 +287,36 @@ def test_trained_surrogate_gains_over_untrained():
   sur.fit(Wb, Gb, epochs=40, batch=min(1024, C * 4), lr=2e-3, lr_final=1e-4)
   a_trained = acceptance()
   assert a_trained > 0.6 and a_trained > a_untrained + 0.1, (a_untrained, a_trained)
+
+
+def test_surrogate_ratio_applies_the_likelihood_weight():
+  """With a likelihood weight w != 1 (edhmc_set_lik_weight) both ends of the
+  surrogate transition's Metropolis ratio carry w * loglik + log prior: the
+  device ratio (edhmc_last_ratio) equals K(r0) - K(rL) + lw(zL) - lw(z0) of
+  the FP64 oracle at the device's own end point and momenta (edhmc_get_momentum),
+  within 1e-6 relative of the log likelihood."""
+  D, C, w = 16, 64, 0.37
+  wl, eng = _engine(N=3000, D=D, C=C, seed=21)
+  sur = _surrogate(eng, H=64, B=C, seed=22)
+  rng = np.random.default_rng(23)
+  m = O.GLM(wl.X.astype(np.float64), wl.y, wl.prior_loc, wl.prior_scale)
+  w_map, cov = O.laplace_logreg(m)
+  sd = np.sqrt(np.diag(cov))
+  _set_norm(sur, {'in_shift': w_map, 'in_scale': 1.0 / sd, 'out_shift': np.zeros(D),
+                  'out_scale': 1.0 / sd})
+  W0 = (w_map + sd * rng.normal(size=(C, D))).astype(np.float32)
+  r0 = rng.normal(size=(C, D)).astype(np.float32)
+  eng.set_lik_weight(w)
+  eng.set_state(W0)
+  sur.transition(0.3 * sd.min(), 5, momenta=r0, uniforms=np.zeros(C))
+  z, _, _ = eng.get_state()
+  zL = z.cpu().numpy().astype(np.float64)
+  rL = eng.last_momentum().cpu().numpy().astype(np.float64)
+  ratio = eng.last_ratio().cpu().numpy()
+
+  def lw(W):
+    return w * m.loglik(W) + O.normal_logp(W, m.prior_loc, m.prior_scale).sum(1)
+  want = (0.5 * (r0.astype(np.float64) ** 2).sum(1) - 0.5 * (rL ** 2).sum(1)
+          + lw(zL) - lw(W0.astype(np.float64)))
+  np.testing.assert_allclose(ratio, want, rtol=0, atol=1e-6 * np.abs(m.loglik(zL)).max() + 1e-6)
+  eng.set_lik_weight(1.0)
