@@ -91,12 +91,14 @@ def parse():
                   help='1: the burn-in step-size centre follows the acceptance rate')
   ap.add_argument('--plain-steps', type=int, default=16, help='timed plain fp32 transitions')
   ap.add_argument('--workload', default='large',
-                  choices=['large', 'hier', 'copy', 'forward'],
+                  choices=['large', 'hier', 'copy', 'forward', 'surrogate'],
                   help="'hier': the hierarchical logistic regression (G=2048 groups of "
                        "1+Poisson(400) rows, D=64, 512 chains, L=16), its own JSON line; "
                        "'copy': 1 GiB device-to-device copies, the byte-counter calibration "
                        "of HBM traffic profiles; 'forward': --steps exact forward passes "
-                       "(the gemm_ll kernel alone) on the large workload, for profiling")
+                       "(the gemm_ll kernel alone) on the large workload, for profiling; "
+                       "'surrogate': --steps surrogate transitions after a short warm start, "
+                       "for profiling")
   ap.add_argument('--hidden', type=int, default=1024)
   ap.add_argument('--epochs', type=int, default=16)
   ap.add_argument('--lr', type=float, default=3e-3)
@@ -510,6 +512,38 @@ def bench_forward(args):
                     'achieved_tflops': flop / (ms / max(n, 1) * 1e-3) / 1e12}))
 
 
+def bench_surrogate(args):
+  """--steps surrogate transitions on the large workload after a short
+  Laplace warm start and two training epochs: the sampling window's kernels
+  (the exact forward and the MLP layers with their leapfrog epilogues) with
+  little else around them, for counter profiles."""
+  import torch
+  from edward_amd import workloads as Wk
+  from edward_amd.engine import GLMEngine
+  from edward_amd.surrogate import Surrogate
+  from edward_amd.warmstart import LaplaceApproximation
+  wl = Wk.logreg_large()
+  C = wl.C
+  eng = GLMEngine(wl.X, wl.y, wl.prior_loc, wl.prior_scale, precision='bf16')
+  lap = LaplaceApproximation(eng)
+  Wp, Gp = lap.gradient_pairs(4 * C, C, seed=args.seed + 1)
+  sur = Surrogate(eng, hidden=args.hidden, max_batch=C, seed=args.seed)
+  sur.fit(Wp, Gp, epochs=2, batch=C, lr=args.lr, lr_final=1e-3, seed=args.seed)
+  eng.set_state(lap.sample(C, seed=args.seed + 2))
+  n_acc = torch.zeros(C, dtype=torch.int32, device='cuda')
+  torch.cuda.synchronize()
+  t0 = time.time()
+  for k in range(args.steps):
+    sur.transition(wl.step_size, wl.n_steps, seed=args.seed, counter=k, n_accept=n_acc)
+  torch.cuda.synchronize()
+  secs = time.time() - t0
+  print(json.dumps({'metric': 'surrogate transitions, N=%d D=%d chains=%d L=%d H=%d'
+                              % (wl.N, wl.D, C, wl.n_steps, args.hidden),
+                    'value': args.steps / secs, 'unit': 'transitions/s', 'steps': args.steps,
+                    'ms_per_transition': secs / args.steps * 1e3,
+                    'accept': float(n_acc.cpu().numpy().mean()) / args.steps}))
+
+
 def main():
   args = parse()
   if args.gpus != 1:
@@ -523,6 +557,8 @@ def main():
     return bench_copy(args)
   if args.workload == 'forward':
     return bench_forward(args)
+  if args.workload == 'surrogate':
+    return bench_surrogate(args)
   import torch
   from edward_amd import workloads as Wk
   from edward_amd.engine import GLMEngine, jittered_step
