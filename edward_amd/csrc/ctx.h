@@ -74,6 +74,7 @@ struct edhmc_ctx {
   int64_t N = 0, N_pad = 0;
   int D = 0, D_pad = 0;
   edhmc::DevBuf Xr, Xc, y, ploc, pscale;
+  edhmc::DevBuf Xf;   // bf16 X in MFMA fragment order (the W-stationary kernels' copy)
   edhmc::DevBuf pkind, plnorm;   // constrained prior families (has_kinds)
   bool has_kinds = false;
 

@@ -400,6 +400,27 @@ layout_x_kernel(const Tin* __restrict__ X, int64_t N, int D, T* __restrict__ Xr,
   }
 }
 
+// Xf: bf16 X in the operand order of v_mfma_f32_16x16x32_bf16, for the
+// W-stationary kernels.  Block (b, s) -- rows 16 b .. 16 b + 15, columns
+// 32 s .. 32 s + 31 -- is 1 KB at element offset (b * D_pad / 32 + s) * 512,
+// and its 16-byte piece l (l < 64) holds row 16 b + (l & 15), columns
+// 32 s + 8 (l >> 4) .. + 8: a wave's operand load for one 16-row block and
+// one 32-column k step is then 1 KB contiguous.  One thread per piece.
+__global__ void __launch_bounds__(256)
+layout_frag_kernel(const bf16_t* __restrict__ Xr, int64_t pieces, int D_pad,
+                   bf16_t* __restrict__ Xf) {
+  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= pieces) return;
+  const int KS = D_pad / 32;
+  const int64_t blk = q >> 6;
+  const int l = (int)(q & 63);
+  const int64_t b = blk / KS;
+  const int s = (int)(blk - b * KS);
+  const int64_t n = 16 * b + (l & 15);
+  const int d = 32 * s + 8 * (l >> 4);
+  *(uint4*)(Xf + q * 8) = *(const uint4*)(Xr + n * D_pad + d);
+}
+
 // Deterministic reduction of the log-likelihood partials llpart[t][c] over
 // the row tiles t, in two fixed-order passes: kLLSplit contiguous ranges of t
 // per 64-chain block (4 interleaved strides inside a range, combined in a
@@ -1149,10 +1170,11 @@ int glm_ensure_chains(edhmc_ctx* ctx, int C);
 // or sixteen consecutive rows hits disjoint banks).  Each wave streams its
 // own 64 rows of X per 512-row tile straight from memory into the operand
 // registers of v_mfma_f32_16x16x32_bf16: for a 16-row block and a 64-byte
-// k step, lane l holds row l & 15, bytes 16 (l >> 4) .. +16, so the four
-// lanes of a row read 64 contiguous bytes (16 rows per load instruction,
-// half the cache lines per byte of the 32 x 32 layout).  kWsPre k steps of X
-// are in flight, the prefetch running on into the next tile while the
+// k step, lane l holds row l & 15, bytes 16 (l >> 4) .. +16.  X is read from
+// Xf, a copy in exactly that order (layout_frag_kernel), so each load
+// instruction reads 1 KB contiguous -- eight whole cache lines, where the
+// row-major copy gives sixteen half lines (12 % of the forward's time).
+// kWsPre k steps of X are in flight, the prefetch running on into the next tile while the
 // previous tile's epilogue runs; no LDS staging of X and no block barrier in
 // the main loop.  Per k step a wave issues 32 MFMAs (2 planes x 4 chain
 // blocks x 4 row blocks).  The epilogue of a tile adds y eta - softplus(eta) over the
@@ -1168,11 +1190,11 @@ constexpr int kWsTile = 8 * kWsWaveRows;   // rows per tile
 #endif
 constexpr int kWsPre = EDHMC_WS_PRE;       // 64-byte k steps of X in flight per wave
 #ifndef EDHMC_WS_PRE_GRAD
-#define EDHMC_WS_PRE_GRAD 2
+#define EDHMC_WS_PRE_GRAD 4
 #endif
 constexpr int kWsPreGrad = EDHMC_WS_PRE_GRAD;   // the same with the residual epilogue
 #ifndef EDHMC_WS_PRE_GRAD1
-#define EDHMC_WS_PRE_GRAD1 2
+#define EDHMC_WS_PRE_GRAD1 4
 #endif
 constexpr int kWsPreGrad1 = EDHMC_WS_PRE_GRAD1; // ... on one plane without the log-likelihood
 constexpr int kWsMaxDPad = 512;
@@ -1192,8 +1214,8 @@ __device__ inline void mfma16(f32x4v_t& acc, const uint4& a, const uint4& b) {
 // WR: also the gradient pass's residuals r = y - sigmoid(eta) (bf16), written
 // to Rt in the K-blocked layout [N_pad / 32][C_pad][32] that GEMM2 reads
 // (0 for the padded rows N .. N_pad), beside the log-likelihood partials.
-// PRE: 64-byte k steps of X in flight per wave (fewer with the residual
-// epilogue, whose registers they would otherwise crowd out).
+// PRE: 64-byte k steps of X in flight per wave (4 in every variant: 2 or 8
+// measured slower, 8 spilling).
 // PLANES: 2 (hi and lo planes of W, the exact log joint) or 1 (hi plane
 // only: the one-plane inner leapfrog gradients); LL: sum y eta - softplus(eta)
 // (bern_nll_fast's non-negative form).
@@ -1237,12 +1259,14 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
   const int KS = D_pad / 32;                 // 64-byte k steps (a multiple of PRE)
   const int64_t n_tiles = n_end > n_begin ? (n_end - n_begin + kWsTile - 1) / kWsTile : 0;
   const int l16 = lane & 15, lq = (lane >> 4) * 16;
-  // byte address of this lane's X fragment row (block j of the wave's rows
-  // in tile t) at k step 0; rows past the padding are clamped (never used)
+  // byte address of this lane's 16-byte piece of the Xf block of row block j
+  // of the wave's rows in tile t at k step 0 (k step ks adds ks KB); blocks
+  // past the padding are clamped (never used)
+  const int64_t last_blk = N_pad / 16 - 1;
   auto row_ptr = [&](int64_t t, int j) -> const char* {
-    int64_t n = n_begin + t * kWsTile + wave * kWsWaveRows + j * 16 + l16;
-    n = n < N_pad ? n : N_pad - 1;
-    return (const char*)(X + n * D_pad) + lq;
+    int64_t b = (n_begin + t * kWsTile + wave * kWsWaveRows + j * 16) >> 4;
+    b = b < last_blk ? b : last_blk;
+    return (const char*)X + (b * KS) * 1024 + lane * 16;
   };
   // Forward (!WR): W is the MFMA's A operand and X its B operand, so
   // acc[i][j][v] is chain 16 i + 4 (lane >> 4) + v at data row 16 j + l16 of
@@ -1266,7 +1290,7 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
 #pragma unroll
       for (int u = 0; u < PRE; ++u)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) xr[u][j] = *(const uint4*)(rp[j] + u * 64);
+        for (int j = 0; j < 4; ++j) xr[u][j] = *(const uint4*)(rp[j] + u * 1024);
     }
     f32x4v_t acc[4][4];
 #pragma unroll
@@ -1293,7 +1317,7 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
       if (tl >= n_tiles) { tl = n_tiles - 1; kl = KS - PRE; }
       const char* rp[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) rp[j] = row_ptr(tl, j) + kl * 64;
+      for (int j = 0; j < 4; ++j) rp[j] = row_ptr(tl, j) + kl * 1024;
       // forward: y of the wave's 64 rows of the next block's tile (lane l
       // holding row l, one coalesced load per k block; padded rows read
       // y = 0).  It is carried into the next iteration (y_cur below), whose
@@ -1323,7 +1347,7 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
             }
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) xr[u][j] = *(const uint4*)(rp[j] + u * 64);
+        for (int j = 0; j < 4; ++j) xr[u][j] = *(const uint4*)(rp[j] + u * 1024);
       }
       if (ks0 + PRE == KS) {
         if (!WR) {
@@ -1450,15 +1474,21 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
   }
 }
 
-// Whether the W-stationary forward applies: Bernoulli, bf16, D_pad <= 512,
-// chain tiles a multiple of 8 (one slice per XCD), enough rows to split.
+// Whether the bound data suit the W-stationary kernels (and bind makes Xf):
+// Bernoulli, bf16, D_pad <= 512, enough rows to split.
+static bool ws_data_ok(const edhmc_ctx* ctx) {
+  return ctx->cdt == EDHMC_BF16 && ctx->lik == EDHMC_LIK_BERNOULLI && ctx->D_pad <= kWsMaxDPad &&
+         (ctx->D_pad / 32) % kWsPre == 0 && (ctx->D_pad / 32) % kWsPreGrad == 0 &&
+         (ctx->D_pad / 32) % kWsPreGrad1 == 0 && ctx->N >= 16 * kWsTile;
+}
+
+// Whether the W-stationary forward applies: the data suit it (Xf made) and
+// the chain tiles are a multiple of 8 (one slice per XCD).
 static bool ws_forward_ok(const edhmc_ctx* ctx) {
 #ifdef EDHMC_NO_WS_FORWARD
   return false;
 #endif
-  return ctx->cdt == EDHMC_BF16 && ctx->lik == EDHMC_LIK_BERNOULLI && ctx->D_pad <= kWsMaxDPad &&
-         (ctx->D_pad / 32) % kWsPre == 0 && ctx->C_pad % (8 * kWsBM) == 0 &&
-         ctx->N >= 16 * kWsTile;
+  return ws_data_ok(ctx) && ctx->C_pad % (8 * kWsBM) == 0;
 }
 
 static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st, bool wr, int planes, bool ll) {
@@ -1482,21 +1512,21 @@ static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st, bool wr, int planes
     if (wr && planes == 1 && !ll)
       hipLaunchKernelGGL((gemm_ll_ws_kernel<true, 1, false, kWsPreGrad1>), dim3(grid),
                          dim3(kWsThreads), 0, st, (const bf16_t*)ctx->z_lp.p,
-                         (int64_t)ctx->C_pad * ctx->D_pad, ctx->D_pad, (const bf16_t*)ctx->Xr.p,
+                         (int64_t)ctx->C_pad * ctx->D_pad, ctx->D_pad, (const bf16_t*)ctx->Xf.p,
                          ctx->N, ctx->N_pad, rows_per_part, parts, mt_per_xcd,
                          part_per_xcd ? 1 : 0, (double*)ctx->llpart.p, (int64_t)ctx->C_pad,
                          (const float*)ctx->y.p, (bf16_t*)ctx->Rt.p, (int64_t)ctx->C_pad);
     else if (wr && planes == 2)
       hipLaunchKernelGGL((gemm_ll_ws_kernel<true, 2, true, kWsPreGrad>), dim3(grid),
                          dim3(kWsThreads), 0, st, (const bf16_t*)ctx->z_lp.p,
-                         (int64_t)ctx->C_pad * ctx->D_pad, ctx->D_pad, (const bf16_t*)ctx->Xr.p,
+                         (int64_t)ctx->C_pad * ctx->D_pad, ctx->D_pad, (const bf16_t*)ctx->Xf.p,
                          ctx->N, ctx->N_pad, rows_per_part, parts, mt_per_xcd,
                          part_per_xcd ? 1 : 0, (double*)ctx->llpart.p, (int64_t)ctx->C_pad,
                          (const float*)ctx->y.p, (bf16_t*)ctx->Rt.p, (int64_t)ctx->C_pad);
     else if (!wr && planes == 2 && ll)
       hipLaunchKernelGGL((gemm_ll_ws_kernel<false, 2, true, kWsPre>), dim3(grid), dim3(kWsThreads), 0, st,
                          (const bf16_t*)ctx->z_lp.p, (int64_t)ctx->C_pad * ctx->D_pad,
-                         ctx->D_pad, (const bf16_t*)ctx->Xr.p, ctx->N, ctx->N_pad,
+                         ctx->D_pad, (const bf16_t*)ctx->Xf.p, ctx->N, ctx->N_pad,
                          rows_per_part, parts, mt_per_xcd, part_per_xcd ? 1 : 0,
                          (double*)ctx->llpart.p, (int64_t)ctx->C_pad, (const float*)ctx->y.p,
                          nullptr, (int64_t)0);
@@ -1766,7 +1796,7 @@ edhmc_ctx::~edhmc_ctx() {
   for (hipEvent_t e : prof.pool) (void)hipEventDestroy(e);
   if (sur) surrogate_destroy(sur);
   if (hier) hier_destroy(hier);
-  DevBuf* bufs[] = {&Xr, &Xc, &y, &ploc, &pscale, &z, &r, &g, &z0, &g0, &z_lp, &logp, &K0,
+  DevBuf* bufs[] = {&Xr, &Xc, &Xf, &y, &ploc, &pscale, &z, &r, &g, &z0, &g0, &z_lp, &logp, &K0,
                     &flag, &ndiv, &ratio, &Rt, &Gpart, &llpart, &ll, &mom_mean, &mom_m2, &tmp_count,
                     &stats_part, &pred_col, &ll_split, &pkind, &plnorm,
                     &newton};
@@ -1862,6 +1892,14 @@ int edhmc_bind_glm(edhmc_ctx* ctx, const void* X, int32_t x_in_dtype, const floa
                          ctx->N_pad, ctx->D_pad, bad);
   }
   EDHMC_TRY(check_launch(ctx, "layout_x"));
+  ctx->Xf.release();
+  if (ws_data_ok(ctx)) {
+    const int64_t pieces = ctx->N_pad * ctx->D_pad / 8;
+    EDHMC_TRY(dev_alloc(ctx, ctx->Xf, (size_t)ctx->N_pad * ctx->D_pad * 2, false));
+    hipLaunchKernelGGL(layout_frag_kernel, dim3((unsigned)((pieces + 255) / 256)), dim3(256), 0,
+                       st, (const bf16_t*)ctx->Xr.p, pieces, ctx->D_pad, (bf16_t*)ctx->Xf.p);
+    EDHMC_TRY(check_launch(ctx, "layout_frag"));
+  }
   auto padc = [&](const float* src, int64_t n, DevBuf& dst, int64_t npad, float fill) {
     hipLaunchKernelGGL(pad_copy_kernel, dim3((unsigned)((npad + 255) / 256)), dim3(256), 0, st,
                        src, n, (float*)dst.p, npad, fill, bad);
