@@ -537,11 +537,19 @@ def bench_surrogate(args):
     sur.transition(wl.step_size, wl.n_steps, seed=args.seed, counter=k, n_accept=n_acc)
   torch.cuda.synchronize()
   secs = time.time() - t0
+  # per-class kernel times over 5 more transitions, every class bracketed
+  eng.profile(True)
+  for k in range(5):
+    sur.transition(wl.step_size, wl.n_steps, seed=args.seed, counter=args.steps + k)
+  torch.cuda.synchronize()
+  per = {kind: ms / 5 for kind, (ms, n) in eng.profile_read().items() if n}
+  eng.profile(False)
   print(json.dumps({'metric': 'surrogate transitions, N=%d D=%d chains=%d L=%d H=%d'
                               % (wl.N, wl.D, C, wl.n_steps, args.hidden),
                     'value': args.steps / secs, 'unit': 'transitions/s', 'steps': args.steps,
                     'ms_per_transition': secs / args.steps * 1e3,
-                    'accept': float(n_acc.cpu().numpy().mean()) / args.steps}))
+                    'accept': float(n_acc.cpu().numpy().mean()) / args.steps,
+                    'per_transition_ms': per}))
 
 
 def main():

@@ -15,12 +15,19 @@ typedef uint16_t bf16_t;
 
 constexpr double kLog2Pi = 1.8378770664093454835606594728112;
 
-// Round-to-nearest-even float -> bf16 (NaN kept quiet).
+// Round-to-nearest-even float -> bf16 (NaN kept NaN).  On the device this is
+// gfx950's v_cvt_pk_bf16_f32, which gives the host formula's result for
+// every finite input (all 2^32 bit patterns compared on an MI355X) and a NaN
+// for every NaN (whose payload may differ from the host's 0x7fc0).
 __device__ __host__ inline bf16_t f32_to_bf16(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_bit_cast(bf16_t, (__bf16)f);
+#else
   uint32_t u = __builtin_bit_cast(uint32_t, f);
   if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16_t)0x7fc0;
   u += 0x7fffu + ((u >> 16) & 1u);
   return (bf16_t)(u >> 16);
+#endif
 }
 
 __device__ __host__ inline float bf16_to_f32(bf16_t h) {

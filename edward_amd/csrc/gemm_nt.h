@@ -28,6 +28,7 @@
 #pragma once
 
 #include <type_traits>
+#include <utility>
 
 #include "common.h"
 
@@ -186,6 +187,29 @@ template <typename Epi, typename Cfg>
 struct EpiLds<Epi, Cfg, std::void_t<decltype(Epi::template lds_bytes<Cfg>())>> {
   static constexpr int value = Epi::template lds_bytes<Cfg>();
 };
+// Epilogue operands loaded before the K loop: an epilogue with a
+// prefetch<Cfg>(pos) member gets its result as apply's third argument, so
+// the loads' latency hides behind the main loop (an epilogue that reads
+// per-element state, e.g. the leapfrog's positions and momenta).
+struct NoPre {};
+template <typename Epi, typename Cfg, typename = void>
+struct EpiPre {
+  __device__ static NoPre load(const Epi&, const EpiPos&) { return NoPre{}; }
+  template <typename Acc>
+  __device__ static void apply(const Epi& e, Acc& acc, const EpiPos& p, const NoPre&) {
+    e.template apply<Cfg>(acc, p);
+  }
+};
+template <typename Epi, typename Cfg>
+struct EpiPre<Epi, Cfg, std::void_t<decltype(std::declval<const Epi&>().template prefetch<Cfg>(
+                            std::declval<const EpiPos&>()))>> {
+  __device__ static auto load(const Epi& e, const EpiPos& p) { return e.template prefetch<Cfg>(p); }
+  template <typename Acc, typename Pre>
+  __device__ static void apply(const Epi& e, Acc& acc, const EpiPos& p, const Pre& q) {
+    e.template apply<Cfg>(acc, p, q);
+  }
+};
+
 template <typename T, typename Cfg, int A_PLANES, typename Epi>
 __global__ void __launch_bounds__(Cfg::kThreads) __attribute__((amdgpu_waves_per_eu(1, EDHMC_GEMM_WPE)))
 gemm_nt_kernel(GemmShape g, Epi epi) {
@@ -279,6 +303,14 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
     const int row = c / CPR, col = c - row * CPR;
     return row * Cfg::RS + col * 16;
   };
+
+  EpiPos pos;
+  pos.m0 = tm * BM + wm * 32 * WM;
+  pos.n0 = tn * BN + wn * 32 * WN;
+  pos.lane = lane; pos.wm = wm; pos.wn = wn; pos.tm = tm; pos.tn = tn; pos.ks = ks;
+  pos.smem = smem;
+  pos.smem_bytes = SMEM;
+  const auto pre = EpiPre<Epi, Cfg>::load(epi, pos);
 
   f32x16_t acc[WM][WN];
 #pragma unroll
@@ -397,13 +429,7 @@ gemm_nt_kernel(GemmShape g, Epi epi) {
 #undef EDHMC_SSTORE
 #undef EDHMC_GLOAD
 
-  EpiPos pos;
-  pos.m0 = tm * BM + wm * 32 * WM;
-  pos.n0 = tn * BN + wn * 32 * WN;
-  pos.lane = lane; pos.wm = wm; pos.wn = wn; pos.tm = tm; pos.tn = tn; pos.ks = ks;
-  pos.smem = smem;
-  pos.smem_bytes = SMEM;
-  epi.template apply<Cfg>(acc, pos);
+  EpiPre<Epi, Cfg>::apply(epi, acc, pos, pre);
 }
 
 // ---------------------------------------------------------------------------

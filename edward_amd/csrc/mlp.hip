@@ -17,6 +17,7 @@
 // Layouts are padded to 128 in every feature dimension and the batch; pads
 // hold zeros, so they contribute nothing (GELU(0) = 0).
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 
 #include "common.h"
@@ -29,8 +30,14 @@
 namespace edhmc {
 
 using CfgM = TileCfg<2, 2, 2, 2>;    // 128 x 128 tiles
-// 64 x 64 tiles with 128-byte K stages for GEMMs with few output tiles (the
-// 4096-chain layers: 1024 workgroups of 4 waves, four resident per CU)
+// 128 x 128 tiles of eight waves (64 x 32 each) and 128-byte K stages, for
+// 256 to 511 such tiles (the 4096-chain hidden layers: one workgroup per CU)
+using CfgM8 = TileCfg<2, 1, 2, 4, 128>;
+// 128 x 64 tiles of eight waves, for 128 to 255 128 x 128 tiles (the
+// 4096-chain output layer, 4096 x 512: 256 workgroups)
+using CfgM8N = TileCfg<1, 1, 4, 2, 128>;
+// 64 x 64 tiles with 128-byte K stages for GEMMs with fewer output tiles
+// (the training pass's weight gradients)
 using CfgMS = TileCfg<1, 1, 2, 2, 128>;
 #ifndef EDHMC_MLP_THRESH
 #define EDHMC_MLP_THRESH 512
@@ -39,19 +46,21 @@ using CfgMS = TileCfg<1, 1, 2, 2, 128>;
 constexpr float kGeluK0 = 0.7978845608028654f;  // sqrt(2 / pi)
 constexpr float kGeluK1 = 0.044715f;
 
-__device__ inline float tanh_fast(float a) {
-  return 1.0f - 2.0f * fast_rcp(__expf(2.0f * a) + 1.0f);
+// sigmoid(2 u) for u = k0 (x + k1 x^3): 1 / (1 + exp(-2 u)), from v_exp_f32
+// and v_rcp_f32 (exp overflowing to +inf for very negative x gives 0).
+__device__ inline float gelu_sig(float x) {
+  const float p = fmaf(kGeluK1 * x * x, x, x);   // u / k0
+  return fast_rcp(1.0f + __builtin_amdgcn_exp2f(p * (-2.0f * kGeluK0 * 1.4426950408889634f)));
 }
 
-// GELU(x) = x/2 (1 + tanh(k0 (x + k1 x^3)))  (Hendrycks & Gimpel 2016).
-__device__ inline float gelu(float x) {
-  const float t = tanh_fast(kGeluK0 * (x + kGeluK1 * x * x * x));
-  return 0.5f * x * (1.0f + t);
-}
+// GELU(x) = x/2 (1 + tanh(k0 (x + k1 x^3)))  (Hendrycks & Gimpel 2016),
+// evaluated as x sigmoid(2 k0 (x + k1 x^3))  (1 + tanh a = 2 sigmoid(2 a)).
+__device__ inline float gelu(float x) { return x * gelu_sig(x); }
 
+// d GELU / dx = s + 2 x s (1 - s) k0 (1 + 3 k1 x^2),  s = sigmoid(2 u).
 __device__ inline float gelu_grad(float x) {
-  const float t = tanh_fast(kGeluK0 * (x + kGeluK1 * x * x * x));
-  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * kGeluK0 * (1.0f + 3.0f * kGeluK1 * x * x);
+  const float sg = gelu_sig(x);
+  return fmaf(2.0f * x * sg * (1.0f - sg), kGeluK0 * fmaf(3.0f * kGeluK1, x * x, 1.0f), sg);
 }
 
 struct Surrogate {
@@ -204,30 +213,75 @@ struct EpiGradOut {
   }
 };
 
-// Leapfrog step fused into the output layer: ghat = surrogate gradient at z;
+// Leapfrog step fused into the output layer: g = surrogate gradient at z;
 // middle steps kick twice by eps/2 and drift, and write the next normalised
-// input (into x0 and into the x columns of the *other* [h2 | x] buffer: the
-// one this GEMM reads is an operand of sibling workgroups still running);
-// the last step kicks by eps/2 and writes the hi/lo position planes for the
-// exact log joint.
+// input into the x columns of the *other* [h2 | x] buffer (the one this GEMM
+// reads is an operand of sibling workgroups still running; the next step's
+// first layer reads its input there); the last step kicks by eps/2, keeps g
+// in ghat (the next trajectory's first half kick) and writes the hi/lo
+// position planes for the exact log joint.  Middle steps do not store g:
+// nothing reads it.
 struct EpiLeapfrog {
   const float* bias; const float* out_shift; const float* out_scale;
   const float* in_shift; const float* in_scale;
   int C, D, Dz;              // chains, dims, row stride of z/r/ghat
-  int Dp;                    // row stride of x0
   float half, full;
   int last;                  // 1: final half kick only
   int eval_only;             // 1: store ghat only (no kick / drift)
   float* z; float* r; float* ghat;
-  bf16_t* x0;                // next MLP input
-  bf16_t* xcat; int xcat_ld; // its copy in the next step's [h2 | x] buffer
+  bf16_t* xcat; int xcat_ld; // next MLP input: the x columns of the next step's [h2 | x]
   bf16_t* zlp; int64_t zlp_plane; int zlp_ld;   // hi/lo planes (last step)
+
+  // z and r of the lane's accumulator elements: element (i, j, rr) is row
+  // m0 + 32 i + acc_row(rr, lane), column n0 + 32 j + (lane & 31); one
+  // outside the C x D state reads element 0 and is never stored.  Loaded
+  // before the K loop (prefetch) where they cost at most 64 registers.
+  template <typename Cfg>
+  struct Pre {
+    float z[Cfg::kWM][Cfg::kWN][16];
+    float r[Cfg::kWM][Cfg::kWN][16];
+  };
+  template <typename Cfg, typename = std::enable_if_t<(Cfg::kWM * Cfg::kWN <= 2)>>
+  __device__ Pre<Cfg> prefetch(const EpiPos& p) const {
+    return load<Cfg>(p);
+  }
   template <typename Cfg>
   __device__ void apply(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p) const {
+    dispatch<Cfg, false>(acc, p, nullptr);
+  }
+  template <typename Cfg>
+  __device__ void apply(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p,
+                        const Pre<Cfg>& q) const {
+    dispatch<Cfg, true>(acc, p, &q);
+  }
+  template <typename Cfg>
+  __device__ Pre<Cfg> load(const EpiPos& p) const {
+    Pre<Cfg> q;
 #pragma unroll
     for (int j = 0; j < Cfg::kWN; ++j) {
       const int n = p.n0 + 32 * j + (p.lane & 31);
-      if (n >= D) continue;
+#pragma unroll
+      for (int i = 0; i < Cfg::kWM; ++i)
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) {
+          const int m = p.m0 + 32 * i + acc_row(rr, p.lane);
+          const int64_t idx = (m < C && n < D) ? (int64_t)m * Dz + n : 0;
+          q.z[i][j][rr] = z[idx];
+          q.r[i][j][rr] = r[idx];
+        }
+    }
+    return q;
+  }
+
+  // MODE 0: store g only; 1: last step; 2: middle step.  CHECK: the wave's
+  // block reaches past C rows or D columns (elements outside are skipped).
+  // PRE: z and r come from q (else they are loaded here).
+  template <typename Cfg, int MODE, bool CHECK, bool PRE>
+  __device__ void step(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p,
+                       const Pre<Cfg>* q) const {
+#pragma unroll
+    for (int j = 0; j < Cfg::kWN; ++j) {
+      const int n = p.n0 + 32 * j + (p.lane & 31);   // < Dp: the per-column vectors' length
       const float bj = bias[n], sc = out_scale[n], sh = out_shift[n];
       const float ish = in_shift[n], isc = in_scale[n];
 #pragma unroll
@@ -235,28 +289,46 @@ struct EpiLeapfrog {
 #pragma unroll
         for (int rr = 0; rr < 16; ++rr) {
           const int m = p.m0 + 32 * i + acc_row(rr, p.lane);
-          if (m >= C) continue;
+          if (CHECK && !(m < C && n < D)) continue;
           const int64_t idx = (int64_t)m * Dz + n;
           const float g = (acc[i][j][rr] + bj) * sc + sh;
-          ghat[idx] = g;
-          if (eval_only) continue;
-          float rv = r[idx] + half * g;
-          if (last) {
-            r[idx] = rv;
-            const float zv = z[idx];
+          const float rold = PRE ? q->r[i][j][rr] : (MODE ? r[idx] : 0.0f);
+          const float zold = PRE ? q->z[i][j][rr] : (MODE ? z[idx] : 0.0f);
+          if (MODE == 0) {
+            ghat[idx] = g;
+          } else if (MODE == 1) {
+            ghat[idx] = g;
+            r[idx] = rold + half * g;
+            const float zv = zold;
             const bf16_t hi = f32_to_bf16(zv);
             zlp[(int64_t)m * zlp_ld + n] = hi;
             zlp[(int64_t)m * zlp_ld + n + zlp_plane] = f32_to_bf16(zv - bf16_to_f32(hi));
           } else {
-            rv = rv + half * g;
-            const float zn = z[idx] + full * rv;
+            const float rv = (rold + half * g) + half * g;
+            const float zn = zold + full * rv;
             r[idx] = rv;
             z[idx] = zn;
-            const bf16_t xb = f32_to_bf16((zn - ish) * isc);
-            x0[(int64_t)m * Dp + n] = xb;
-            xcat[(int64_t)m * xcat_ld + n] = xb;
+            xcat[(int64_t)m * xcat_ld + n] = f32_to_bf16((zn - ish) * isc);
           }
         }
+    }
+  }
+
+  template <typename Cfg, bool PRE>
+  __device__ void dispatch(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p,
+                           const Pre<Cfg>* q) const {
+    // without the prefetch the checked form keeps the loads next to their
+    // use (batching them all would double the registers of large tiles)
+    const bool check = !PRE || p.m0 + 32 * Cfg::kWM > C || p.n0 + 32 * Cfg::kWN > D;
+    if (eval_only) {
+      if (check) step<Cfg, 0, true, PRE>(acc, p, q);
+      else step<Cfg, 0, false, PRE>(acc, p, q);
+    } else if (last) {
+      if (check) step<Cfg, 1, true, PRE>(acc, p, q);
+      else step<Cfg, 1, false, PRE>(acc, p, q);
+    } else {
+      if (check) step<Cfg, 2, true, PRE>(acc, p, q);
+      else step<Cfg, 2, false, PRE>(acc, p, q);
     }
   }
 };
@@ -510,14 +582,19 @@ static int mlp_gemm_cfg(edhmc_ctx* ctx, const void* A, int64_t lda_elems, const 
   return check_launch(ctx, "mlp_gemm");
 }
 
-// 128 x 128 tiles, or 64 x 64 when that leaves fewer than two workgroups per
-// CU (e.g. the output layer: 4096 x 512 is only 128 large tiles).
+// The tile shape by the number of 128 x 128 output tiles (M and N are
+// multiples of 128).  Measured on the 4096-chain layers (MI355X, us per
+// launch, 64 x 64 / chosen): 4096 x 1024 x 512  13.8 / 9.8,
+// 4096 x 1024 x 1024  19.2 / 14.4, 4096 x 512 x 1536  14.1 / 12.8.
 template <typename Epi>
 static int mlp_gemm(edhmc_ctx* ctx, const void* A, int64_t lda_elems, const void* B,
                     int64_t ldb_elems, int M, int N, int K, Epi e, hipStream_t st) {
-  if ((M / CfgM::BM) * (N / CfgM::BN) < EDHMC_MLP_THRESH)
-    return mlp_gemm_cfg<CfgMS>(ctx, A, lda_elems, B, ldb_elems, M, N, K, e, st);
-  return mlp_gemm_cfg<CfgM>(ctx, A, lda_elems, B, ldb_elems, M, N, K, e, st);
+  const int tiles = (M / 128) * (N / 128);
+  if (tiles >= EDHMC_MLP_THRESH)
+    return mlp_gemm_cfg<CfgM>(ctx, A, lda_elems, B, ldb_elems, M, N, K, e, st);
+  if (tiles >= 256) return mlp_gemm_cfg<CfgM8>(ctx, A, lda_elems, B, ldb_elems, M, N, K, e, st);
+  if (tiles >= 128) return mlp_gemm_cfg<CfgM8N>(ctx, A, lda_elems, B, ldb_elems, M, N, K, e, st);
+  return mlp_gemm_cfg<CfgMS>(ctx, A, lda_elems, B, ldb_elems, M, N, K, e, st);
 }
 
 static int transpose(edhmc_ctx* ctx, const DevBuf& in, int R, int Cc, DevBuf& out, hipStream_t st) {
@@ -543,11 +620,12 @@ static int refresh_weights(edhmc_ctx* ctx, Surrogate* s, hipStream_t st) {
   return EDHMC_OK;
 }
 
-// Forward through the two hidden layers for the batch in s->x0 (Bp rows);
-// keep_a: store pre-activations for the backward pass; h2 goes into the
-// first Hp columns of h2_out ([Bp x Kc]; s->h2 unless given).
+// Forward through the two hidden layers for the batch in s->x0 (Bp rows),
+// or in xcat's x columns; keep_a: store pre-activations for the backward
+// pass; h2 goes into the first Hp columns of xcat ([Bp x Kc]; s->h2 unless
+// given).
 static int forward_hidden(edhmc_ctx* ctx, Surrogate* s, int Bp, bool keep_a, hipStream_t st,
-                          bf16_t* h2_out = nullptr) {
+                          bf16_t* xcat = nullptr, bool x_from_xcat = false) {
   const float* P = (const float*)s->P.p;
   EpiBiasGelu e1;
   e1.bias = P + s->ob1;
@@ -555,11 +633,15 @@ static int forward_hidden(edhmc_ctx* ctx, Surrogate* s, int Bp, bool keep_a, hip
   e1.h_out = (bf16_t*)s->h1.p;
   e1.ld = s->Hp;
   e1.ldh = s->Hp;
-  EDHMC_TRY(mlp_gemm(ctx, s->x0.p, s->Dp, s->W1b.p, s->Dp, Bp, s->Hp, s->Dp, e1, st));
+  bf16_t* xc = xcat ? xcat : (bf16_t*)s->h2.p;
+  if (x_from_xcat)
+    EDHMC_TRY(mlp_gemm(ctx, xc + s->Hp, s->Kc, s->W1b.p, s->Dp, Bp, s->Hp, s->Dp, e1, st));
+  else
+    EDHMC_TRY(mlp_gemm(ctx, s->x0.p, s->Dp, s->W1b.p, s->Dp, Bp, s->Hp, s->Dp, e1, st));
   EpiBiasGelu e2;
   e2.bias = P + s->ob2;
   e2.a_out = keep_a ? (float*)s->a2.p : nullptr;
-  e2.h_out = h2_out ? h2_out : (bf16_t*)s->h2.p;
+  e2.h_out = xc;
   e2.ld = s->Hp;
   e2.ldh = s->Kc;
   return mlp_gemm(ctx, s->h1.p, s->Hp, s->W2b.p, s->Hp, Bp, s->Hp, s->Hp, e2, st);
@@ -648,15 +730,14 @@ static int eval_ghat(edhmc_ctx* ctx, Surrogate* s, hipStream_t st);
 
 // Per-chain begin of a surrogate trajectory.
 struct SurChainArgs {
-  int C, D, Dz, Dp;
+  int C, D, Dz;
   float* z; float* r; float* z0; float* ghat; float* ghat0;
   double* K0;
   const float* r_fed;
   uint64_t seed, counter;
   float half, full;
   const float* in_shift; const float* in_scale;
-  bf16_t* x0;
-  bf16_t* xcat; int xcat_ld;
+  bf16_t* xcat; int xcat_ld;   // first MLP input: the x columns of step 1's [h2 | x]
 };
 
 __global__ void __launch_bounds__(256) sur_begin_kernel(SurChainArgs a) {
@@ -689,9 +770,7 @@ __global__ void __launch_bounds__(256) sur_begin_kernel(SurChainArgs a) {
       const float zn = zv + a.full * r;
       a.z[idx] = zn;
       a.r[idx] = r;
-      const bf16_t xb = f32_to_bf16((zn - a.in_shift[d]) * a.in_scale[d]);
-      a.x0[(int64_t)c * a.Dp + d] = xb;
-      a.xcat[(int64_t)c * a.xcat_ld + d] = xb;
+      a.xcat[(int64_t)c * a.xcat_ld + d] = f32_to_bf16((zn - a.in_shift[d]) * a.in_scale[d]);
     }
   }
   k = block_sum_f64<256>(k, red);
@@ -783,7 +862,6 @@ static EpiLeapfrog lf_epi(edhmc_ctx* ctx, Surrogate* s, float half, float full, 
   e.C = ctx->C;
   e.D = ctx->D;
   e.Dz = ctx->D_pad;
-  e.Dp = s->Dp;
   e.half = half;
   e.full = full;
   e.last = last;
@@ -791,7 +869,6 @@ static EpiLeapfrog lf_epi(edhmc_ctx* ctx, Surrogate* s, float half, float full, 
   e.z = (float*)ctx->z.p;
   e.r = (float*)ctx->r.p;
   e.ghat = (float*)s->ghat.p;
-  e.x0 = (bf16_t*)s->x0.p;
   e.xcat = xbuf(s, k + 1) + s->Hp;
   e.xcat_ld = s->Kc;
   e.zlp = (bf16_t*)ctx->z_lp.p;
@@ -825,7 +902,7 @@ static int eval_ghat(edhmc_ctx* ctx, Surrogate* s, hipStream_t st) {
                      (bf16_t*)s->x0.p, Bp, s->Dp,
                      (bf16_t*)s->h2.p + s->Hp, s->Kc);
   EDHMC_TRY(check_launch(ctx, "norm_in"));
-  EDHMC_TRY(forward_hidden(ctx, s, Bp, false, st));
+  EDHMC_TRY(forward_hidden(ctx, s, Bp, false, st, nullptr, true));
   EpiLeapfrog e = lf_epi(ctx, s, 0.0f, 0.0f, 0, 1, 0);
   EDHMC_TRY(mlp_gemm(ctx, s->h2.p, s->Kc, s->W3b.p, s->Kc, Bp, s->Dp, s->Kc, e, st));
   s->ghat_valid = true;
@@ -847,14 +924,13 @@ static int sur_transition(edhmc_ctx* ctx, float eps, int L, const float* r0, con
   const float half = (float)(0.5 * (double)eps), full = eps;
   const float* nrm = (const float*)s->nrm.p;
   SurChainArgs a;
-  a.C = ctx->C; a.D = ctx->D; a.Dz = ctx->D_pad; a.Dp = s->Dp;
+  a.C = ctx->C; a.D = ctx->D; a.Dz = ctx->D_pad;
   a.z = (float*)ctx->z.p; a.r = (float*)ctx->r.p; a.z0 = (float*)ctx->z0.p;
   a.ghat = (float*)s->ghat.p; a.ghat0 = (float*)s->ghat0.p;
   a.K0 = (double*)ctx->K0.p;
   a.r_fed = r0; a.seed = seed; a.counter = counter;
   a.half = half; a.full = full;
   a.in_shift = nrm; a.in_scale = nrm + s->Dp;
-  a.x0 = (bf16_t*)s->x0.p;
   a.xcat = xbuf(s, 1) + s->Hp;
   a.xcat_ld = s->Kc;
   {
@@ -865,7 +941,7 @@ static int sur_transition(edhmc_ctx* ctx, float eps, int L, const float* r0, con
   const int Bp = (int)round_up(ctx->C, 128);
   for (int k = 1; k <= L; ++k) {
     bf16_t* xk = xbuf(s, k);
-    EDHMC_TRY(forward_hidden(ctx, s, Bp, false, st, xk));
+    EDHMC_TRY(forward_hidden(ctx, s, Bp, false, st, xk, true));
     EpiLeapfrog e = lf_epi(ctx, s, half, full, k == L ? 1 : 0, 0, k);
     EDHMC_TRY(mlp_gemm(ctx, xk, s->Kc, s->W3b.p, s->Kc, Bp, s->Dp, s->Kc, e, st));
   }
