@@ -1474,6 +1474,211 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Pipelined W-stationary forward (the exact log joint of the hi/lo planes
+// for D_pad = 32 KS): four waves per workgroup, one per SIMD, each with the
+// whole tile body unrolled (KS k steps of 64 bytes).  A wave computes the
+// MFMAs of tile t while it evaluates the log-likelihood terms of tile t - 1,
+// 16 / KS (chain block, row block) pairs per k step, from a second
+// accumulator set -- the epilogue's exp / log run between the matrix-core
+// instructions of the same wave instead of after them.  W fragments are read
+// from LDS one k step ahead; X (from Xf) PRE k steps ahead, running on into
+// the next tile.  Per element the arithmetic and its order equal
+// gemm_ll_ws_kernel<false, 2, true, .>'s (rows are split over 4 waves of 64
+// rows per 256-row tile, so the FP64 partial sums group rows differently).
+// ---------------------------------------------------------------------------
+constexpr int kWpThreads = 256;
+constexpr int kWpTile = 4 * kWsWaveRows;   // rows per tile
+
+template <int KS, int PRE>
+__global__ void __launch_bounds__(kWpThreads) __attribute__((amdgpu_waves_per_eu(1, 1)))
+gemm_ll_wp_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems,
+                  const bf16_t* __restrict__ X, int64_t N, int64_t N_pad, int64_t rows_per_part,
+                  int parts, int mt_per_xcd, int part_per_xcd, double* __restrict__ llpart,
+                  int64_t ll_ld, const float* __restrict__ y) {
+  static_assert(KS % PRE == 0 && KS <= 16, "k steps per tile");
+  constexpr int D_pad = 32 * KS;
+  constexpr int RSB = 2 * D_pad + kWsPad;
+  __shared__ __align__(16) char lds[2 * kWsBM * RSB];
+  const int id = blockIdx.x, xcd = id & 7, local = id >> 3;
+  int tm, part;
+  if (part_per_xcd) {
+    part = xcd % parts;
+    tm = (xcd / parts) * mt_per_xcd + local;
+  } else {
+    tm = xcd * mt_per_xcd + local % mt_per_xcd;
+    part = local / mt_per_xcd;
+  }
+  if (part >= parts) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int CPR = D_pad / 8;
+  for (int q = tid; q < 2 * kWsBM * CPR; q += kWpThreads) {
+    const int row = q / CPR, ch = q - row * CPR;
+    const int pl = row / kWsBM, c = row - pl * kWsBM;
+    const uint4 v = *(const uint4*)(Wp + pl * plane_elems + (int64_t)(tm * kWsBM + c) * D_pad +
+                                    ch * 8);
+    *(uint4*)(lds + row * RSB + ch * 16) = v;
+  }
+  __syncthreads();
+
+  const int64_t n_begin = (int64_t)part * rows_per_part;
+  const int64_t n_end = min(n_begin + rows_per_part, N);
+  const int64_t n_tiles = n_end > n_begin ? (n_end - n_begin + kWpTile - 1) / kWpTile : 0;
+  const int l16 = lane & 15, lq = (lane >> 4) * 16;
+  const int64_t last_blk = N_pad / 16 - 1;
+  // this lane's 16-byte piece of the Xf block of row block j of the wave's
+  // rows in tile t at k step 0 (blocks past the padding clamped, never used)
+  auto blk_ptr = [&](int64_t t, int j) -> const char* {
+    int64_t b = (n_begin + t * kWpTile + wave * kWsWaveRows + j * 16) >> 4;
+    b = b < last_blk ? b : last_blk;
+    return (const char*)X + (b * KS) * 1024 + lane * 16;
+  };
+  auto y_at = [&](int64_t t) -> float {
+    const int64_t n = n_begin + t * kWpTile + wave * kWsWaveRows + lane;
+    return y[n < N_pad ? n : N_pad - 1];
+  };
+  double lls[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) lls[i][v] = 0.0;
+  if (n_tiles > 0) {
+    uint4 xr[PRE][4];
+#pragma unroll
+    for (int u = 0; u < PRE; ++u)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) xr[u][j] = *(const uint4*)(blk_ptr(0, j) + u * 1024);
+    f32x4v_t acc[4][4], accE[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        acc[i][j] = f32x4v_t{0.0f, 0.0f, 0.0f, 0.0f};
+        accE[i][j] = f32x4v_t{0.0f, 0.0f, 0.0f, 0.0f};
+      }
+    // W fragments of k step ks (both planes, 4 chain blocks), double-buffered
+    uint4 wf[2][2][4];
+    auto read_w = [&](int buf, int ks) {
+#pragma unroll
+      for (int pl = 0; pl < 2; ++pl)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          wf[buf][pl][i] = *(const uint4*)(lds + (pl * kWsBM + i * 16 + l16) * RSB + ks * 64 + lq);
+    };
+    read_w(0, 0);
+    float y_cur = y_at(0), yE = 0.0f;
+    int64_t tE = -1;   // tile whose log-likelihood terms are pending (none yet)
+    for (int64_t t = 0; t < n_tiles; ++t) {
+      const int64_t tn = t + 1 < n_tiles ? t + 1 : n_tiles - 1;
+      const float y_nx = y_at(tn);
+      const char* pc[4];
+      const char* pn[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        pc[j] = blk_ptr(t, j);
+        pn[j] = blk_ptr(tn, j);
+      }
+      // tile tE's rows of this lane: 16 j + l16 of the wave's rows
+      const int64_t nbE = n_begin + tE * kWpTile + wave * kWsWaveRows + l16;
+      bool validE[4];
+      float yEv[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        validE[j] = tE >= 0 && nbE + 16 * j < n_end;
+        yEv[j] = __shfl(yE, 16 * j + l16, 64);
+      }
+      float s32[4];
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const int b = ks & 1;
+        read_w(b ^ 1, (ks + 1) % KS);   // W is stationary: k step 0 again after the last
+#pragma unroll
+        for (int pl = 0; pl < 2; ++pl)
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) mfma16(acc[i][j], wf[b][pl][i], xr[ks % PRE][j]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          xr[ks % PRE][j] = ks + PRE < KS ? *(const uint4*)(pc[j] + (ks + PRE) * 1024)
+                                          : *(const uint4*)(pn[j] + (ks + PRE - KS) * 1024);
+        // log-likelihood terms of tile tE: (chain block, row block) pairs
+        // [16 ks / KS, 16 (ks + 1) / KS), row blocks innermost
+#pragma unroll
+        for (int pq = 16 * ks / KS; pq < 16 * (ks + 1) / KS; ++pq) {
+          const int i = pq >> 2, j = pq & 3;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            if (j == 0) s32[v] = 0.0f;
+            const float nl = bern_nll_fast(accE[i][j][v], yEv[j]);
+            s32[v] -= validE[j] ? nl : 0.0f;
+            if (j == 3) lls[i][v] += (double)s32[v];
+          }
+        }
+        // keep each k step's loads in their step: hoisting them across the
+        // unrolled body would outgrow the register file
+        __builtin_amdgcn_sched_barrier(0);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          accE[i][j] = acc[i][j];
+          acc[i][j] = f32x4v_t{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+      yE = y_cur;
+      y_cur = y_nx;
+      tE = t;
+    }
+    // the last tile's terms
+    const int64_t nbE = n_begin + tE * kWpTile + wave * kWsWaveRows + l16;
+    bool validE[4];
+    float yEv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      validE[j] = nbE + 16 * j < n_end;
+      yEv[j] = __shfl(yE, 16 * j + l16, 64);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        float s32 = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float nl = bern_nll_fast(accE[i][j][v], yEv[j]);
+          s32 -= validE[j] ? nl : 0.0f;
+        }
+        lls[i][v] += (double)s32;
+      }
+  }
+  // lanes with equal lane >> 4 hold the same chains at different rows
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double x = lls[i][v];
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) x += __shfl_xor(x, off, 64);
+      lls[i][v] = x;
+    }
+  __syncthreads();   // the W tiles are no longer read: reuse LDS for the wave sums
+  double* red = reinterpret_cast<double*>(lds);   // [4 waves][64 chains]
+  if (l16 == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) red[wave * kWsBM + 16 * i + (lane >> 4) * 4 + v] = lls[i][v];
+  }
+  __syncthreads();
+  if (tid < kWsBM) {
+    double x = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWpThreads / 64; ++w) x += red[w * kWsBM + tid];
+    llpart[(int64_t)part * ll_ld + tm * kWsBM + tid] = x;
+  }
+}
+
 // Whether the bound data suit the W-stationary kernels (and bind makes Xf):
 // Bernoulli, bf16, D_pad <= 512, enough rows to split.
 static bool ws_data_ok(const edhmc_ctx* ctx) {
@@ -1523,6 +1728,12 @@ static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st, bool wr, int planes
                          ctx->N, ctx->N_pad, rows_per_part, parts, mt_per_xcd,
                          part_per_xcd ? 1 : 0, (double*)ctx->llpart.p, (int64_t)ctx->C_pad,
                          (const float*)ctx->y.p, (bf16_t*)ctx->Rt.p, (int64_t)ctx->C_pad);
+    else if (!wr && planes == 2 && ll && ctx->D_pad == 512)
+      hipLaunchKernelGGL((gemm_ll_wp_kernel<16, 4>), dim3(grid), dim3(kWpThreads), 0, st,
+                         (const bf16_t*)ctx->z_lp.p, (int64_t)ctx->C_pad * ctx->D_pad,
+                         (const bf16_t*)ctx->Xf.p, ctx->N, ctx->N_pad, rows_per_part, parts,
+                         mt_per_xcd, part_per_xcd ? 1 : 0, (double*)ctx->llpart.p,
+                         (int64_t)ctx->C_pad, (const float*)ctx->y.p);
     else if (!wr && planes == 2 && ll)
       hipLaunchKernelGGL((gemm_ll_ws_kernel<false, 2, true, kWsPre>), dim3(grid), dim3(kWsThreads), 0, st,
                          (const bf16_t*)ctx->z_lp.p, (int64_t)ctx->C_pad * ctx->D_pad,
