@@ -4,8 +4,8 @@ surrogate-driven bf16 HMC versus plain fp32 HMC.
 Workload (default): N = 1e6 rows, D = 512 features (X ~ N(0,1) with column
 scales U(0.5, 2), stored bf16), 4096 chains from N(0, I), eps = 5e-4,
 L = 32 leapfrog steps; the surrogate MLP 512 -> 1024 -> 1024 -> 512 (+ linear
-skip) trained on ~254k (w, grad log p) pairs from the plain sampler's own
-warm-up trajectories -- the large logistic-regression configuration with the
+skip) trained on ~200k (w, grad log p) pairs (196,608) from the plain
+sampler's own warm-up trajectories -- the large logistic-regression configuration with the
 learned-surrogate recipe.  A step is one HMC transition of all chains.
 
 Flow (one JSON line on stdout at the end; progress on stderr):
@@ -104,6 +104,9 @@ def parse():
   ap.add_argument('--lr', type=float, default=3e-3)
   ap.add_argument('--rec-stride', type=int, default=1)
   ap.add_argument('--rec-transitions', type=int, default=2)
+  ap.add_argument('--rec-steps', type=int, default=25,
+                  help='leapfrog steps of the recording transitions (0: the sampler\'s L); '
+                       '2 x 24 inner points x 4096 chains = 196,608 pairs')
   ap.add_argument('--surrogate-only', action='store_true',
                   help='stop after the surrogate phase (tuning runs)')
   ap.add_argument('--seed', type=int, default=0)
@@ -241,7 +244,7 @@ def laplace_variant(bf, wl, args, K):
   warm_s = time.time() - t0
   sur = Surrogate(bf, hidden=args.hidden, max_batch=C, seed=args.seed)
   # 32 epochs over its 114,688 pairs: about as many Adam steps as the
-  # recipe's 16 epochs over ~254k
+  # recipe's 16 epochs over ~200k
   sur.fit(Wp, Gp, epochs=32, batch=C, lr=args.lr, lr_final=5e-5, seed=args.seed,
           val_frac=0.05)
   torch.cuda.synchronize()
@@ -631,8 +634,9 @@ def main():
   means = []
   rec_stride = args.rec_stride
   n_rec = args.rec_transitions
+  Lr = args.rec_steps if args.rec_steps > 0 else L
   if not pairs_from_laplace:
-    cap = ((L - 1) // rec_stride) * C * n_rec
+    cap = ((Lr - 1) // rec_stride) * C * n_rec
     Wrec = torch.empty(cap, D, device=dev)
     Grec = torch.empty(cap, D, device=dev)
   t = 0
@@ -665,11 +669,11 @@ def main():
     for k in range(n_rec):
       eps_t = jittered_step(eps, args.jitter, args.seed, t)
       n_acc.zero_()
-      bf.transition(eps_t, L, seed=args.seed, counter=t, n_accept=n_acc)
+      bf.transition(eps_t, Lr, seed=args.seed, counter=t, n_accept=n_acc)
       _, lp, _ = bf.get_state()
       means.append(float(lp.cpu().numpy().mean()))
       log('recording %d eps %.2e L %d mean logp %.2f accept %.3f'
-          % (t, eps_t, L, means[-1], float(n_acc.cpu().numpy().mean())))
+          % (t, eps_t, Lr, means[-1], float(n_acc.cpu().numpy().mean())))
       t += 1
     bf.set_inner_gradient(False)
   torch.cuda.synchronize()
@@ -769,8 +773,8 @@ def main():
   plain_s_per_t = plain_s / Kp
   # the plain method's own warm-up from N(0, I), priced at its measured
   # seconds per L-step transition over the gradient evaluations the bf16
-  # warm-up took (descent transitions of --burn-in-steps, then n_rec of L)
-  warm_equiv = n_descent * args.burn_in_steps / float(L) + n_rec
+  # warm-up took (descent transitions of --burn-in-steps, then n_rec of Lr)
+  warm_equiv = (n_descent * args.burn_in_steps + n_rec * Lr) / float(L)
   plain_incl_derived = es_plain['min_ess_mixing'] / (plain_s + warm_equiv * plain_s_per_t)
 
   # roofline of the dominant kernel of the surrogate transitions (dominance
@@ -836,7 +840,7 @@ def main():
                        'L=%d with exact gradients recording %d pairs)'
                        % ('Laplace start (device Newton steps) + ' if laplace_info else '',
                           'N(mode, P^-1) draws' if laplace_info else 'N(0, I)', n_descent,
-                          args.burn_in_steps, n_rec, L, n_pairs)) +
+                          args.burn_in_steps, n_rec, Lr, n_pairs)) +
                       ' + surrogate training + %d surrogate transitions: value = mixing '
                       'min-ESS of the %d samples / all of it' % (K, K),
       'warm_start': laplace_info if laplace_info else 'N(0, I) draws, plain bf16 warm-up',

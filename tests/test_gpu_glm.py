@@ -307,11 +307,13 @@ def test_device_ess_matches_oracle(between):
   np.testing.assert_allclose(dev, ref, rtol=1e-6)
 
 
-@pytest.mark.parametrize('N,D,C', [(40000, 200, 512), (20000, 512, 1024), (30000, 128, 2048),
-                                   (30000, 128, 4096)])
+@pytest.mark.parametrize('N,D,C', [(40000, 200, 512), (20000, 512, 1024), (20011, 500, 512),
+                                   (30000, 128, 2048), (30000, 128, 4096)])
 def test_forward_only_logp_chain_stationary_path(N, D, C):
   """Sizes that take the W-stationary kernel (bf16, Bernoulli, chain tiles a
-  multiple of 8, D_pad <= 512; the last two split the rows into 8 and 4
+  multiple of 8, D_pad <= 512; D_pad = 512 takes its pipelined four-wave
+  form, the second and third cases with a ragged last tile and the third
+  with padded columns; the last two split the rows into 8 and 4
   parts, one per XCD or pair of XCDs), forward-only and with the residuals
   of the gradient pass: log joint against the oracle (rtol 1e-5, X rounded
   to bf16 on both sides) and against the log joint of edhmc_logp_grad;
@@ -368,13 +370,15 @@ def test_plain_bf16_transition_on_the_chain_stationary_path():
 
 
 @pytest.mark.parametrize('N,D,C,path', [(20000, 128, 512, 'w-stationary'),
+                                        (20000, 512, 512, 'w-stationary'),
                                         (20000, 640, 256, 'lds-staged'),
                                         (6000, 96, 130, 'lds-staged')])
 def test_plain_bf16_transition_follows_the_rounded_oracle(N, D, C, path):
   """A plain bf16 transition against the oracle's leapfrog with the gradient
   pass's own roundings (GLM.grad_bf16: hi + lo position planes at the ends,
   the hi plane for the inner steps, bf16 residuals), on the W-stationary
-  kernels (bf16, D_pad <= 512, enough rows) and on the LDS-staged GEMMs
+  kernels (bf16, D_pad <= 512, enough rows; D = 128 and 512, the latter in
+  20 row parts with a ragged last tile) and on the LDS-staged GEMMs
   (D_pad = 640 > 512; N below 16 row tiles): two runs bitwise equal; end
   positions of every chain: median within 1e-5 relative (measured 1e-7 to
   7e-7), every chain within 1e-3 (measured up to 1.6e-4 on isolated chains
