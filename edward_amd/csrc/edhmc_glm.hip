@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <mutex>
+#include <type_traits>
 
 #include "common.h"
 #include "ctx.h"
@@ -1731,13 +1732,25 @@ static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st, bool wr, int planes
                          ctx->N, ctx->N_pad, rows_per_part, parts, mt_per_xcd,
                          part_per_xcd ? 1 : 0, (double*)ctx->llpart.p, (int64_t)ctx->C_pad,
                          (const float*)ctx->y.p, (bf16_t*)ctx->Rt.p, (int64_t)ctx->C_pad);
-    else if (!wr && planes == 2 && ll && ctx->D_pad == 512)
-      hipLaunchKernelGGL((gemm_ll_wp_kernel<16, 4>), dim3(grid), dim3(kWpThreads), 0, st,
-                         (const bf16_t*)ctx->z_lp.p, (int64_t)ctx->C_pad * ctx->D_pad,
-                         (const bf16_t*)ctx->Xf.p, ctx->N, ctx->N_pad, rows_per_part, parts,
-                         mt_per_xcd, part_per_xcd ? 1 : 0, (double*)ctx->llpart.p,
-                         (int64_t)ctx->C_pad, (const float*)ctx->y.p);
-    else if (!wr && planes == 2 && ll)
+    else if (!wr && planes == 2 && ll && ctx->D_pad % 128 == 0 && ctx->D_pad >= 256) {
+      // the pipelined forward, its tile body unrolled for D_pad = 32 KS (at
+      // 4096 chains, N = 1e6: 3.64 / 4.93 / 6.1 ms against the eight-wave
+      // kernel's 3.71 / 5.12 / 6.6 for D_pad = 256 / 384 / 512; at 128 the
+      // eight-wave kernel is 2 % faster and keeps it)
+      auto wp = [&](auto ks) {
+        constexpr int KS = decltype(ks)::value;
+        hipLaunchKernelGGL((gemm_ll_wp_kernel<KS, 4>), dim3(grid), dim3(kWpThreads), 0, st,
+                           (const bf16_t*)ctx->z_lp.p, (int64_t)ctx->C_pad * ctx->D_pad,
+                           (const bf16_t*)ctx->Xf.p, ctx->N, ctx->N_pad, rows_per_part, parts,
+                           mt_per_xcd, part_per_xcd ? 1 : 0, (double*)ctx->llpart.p,
+                           (int64_t)ctx->C_pad, (const float*)ctx->y.p);
+      };
+      switch (ctx->D_pad) {
+        case 256: wp(std::integral_constant<int, 8>{}); break;
+        case 384: wp(std::integral_constant<int, 12>{}); break;
+        default: wp(std::integral_constant<int, 16>{}); break;   // 512 (ws_data_ok)
+      }
+    } else if (!wr && planes == 2 && ll)
       hipLaunchKernelGGL((gemm_ll_ws_kernel<false, 2, true, kWsPre>), dim3(grid), dim3(kWsThreads), 0, st,
                          (const bf16_t*)ctx->z_lp.p, (int64_t)ctx->C_pad * ctx->D_pad,
                          ctx->D_pad, (const bf16_t*)ctx->Xf.p, ctx->N, ctx->N_pad,

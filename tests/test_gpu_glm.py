@@ -308,12 +308,12 @@ def test_device_ess_matches_oracle(between):
 
 
 @pytest.mark.parametrize('N,D,C', [(40000, 200, 512), (20000, 512, 1024), (20011, 500, 512),
-                                   (30000, 128, 2048), (30000, 128, 4096)])
+                                   (20000, 384, 512), (30000, 128, 2048), (30000, 128, 4096)])
 def test_forward_only_logp_chain_stationary_path(N, D, C):
   """Sizes that take the W-stationary kernel (bf16, Bernoulli, chain tiles a
-  multiple of 8, D_pad <= 512; D_pad = 512 takes its pipelined four-wave
-  form, the second and third cases with a ragged last tile and the third
-  with padded columns; the last two split the rows into 8 and 4
+  multiple of 8, D_pad <= 512; the forward takes its pipelined four-wave
+  form for D_pad = 256, 384 and 512, the eight-wave one for 128; ragged last tiles,
+  padded columns at D = 200 and 500; the last two split the rows into 8 and 4
   parts, one per XCD or pair of XCDs), forward-only and with the residuals
   of the gradient pass: log joint against the oracle (rtol 1e-5, X rounded
   to bf16 on both sides) and against the log joint of edhmc_logp_grad;
