@@ -68,7 +68,12 @@ const char* edhmc_last_error(const edhmc_ctx* ctx);
  *   compute_dtype  EDHMC_F32: fp32 X, fp32 MFMA;  EDHMC_BF16: X rounded to
  *                  bf16 (RNE), bf16 MFMA with fp32 accumulation.
  * Non-finite X raises EDHMC_ENONFINITE like ed.dot's verify_tensor_all_finite
- * (edward/util/tensorflow.py:33-36).  Rebinding discards chain state. */
+ * (edward/util/tensorflow.py:33-36).  Rebinding discards chain state.
+ * Device memory for the data: X is kept in two layouts (row-major and
+ * K-blocked), and with bf16, a Bernoulli likelihood, D_pad (D rounded up to
+ * 64) a multiple of 128 up to 512 and N >= 8192 in a third,
+ * MFMA-operand-ordered one -- 3 x N x D_pad x 2 bytes (3 GB at N = 1e6,
+ * D = 512). */
 int edhmc_bind_glm(edhmc_ctx* ctx, const void* X, int32_t x_in_dtype, const float* y,
                    int64_t N, int32_t D, int32_t likelihood, float lik_scale,
                    const float* prior_loc, const float* prior_scale,
