@@ -30,6 +30,15 @@ __device__ __host__ inline bf16_t f32_to_bf16(float f) {
 #endif
 }
 
+// Two floats to the bf16 pair (lo | hi << 16), as f32_to_bf16 of each; on
+// the device one v_cvt_pk_bf16_f32.
+__device__ inline uint32_t f32x2_to_bf16x2(float lo, float hi) {
+  typedef __attribute__((ext_vector_type(2))) float f2_t;
+  typedef __attribute__((ext_vector_type(2))) __bf16 b2_t;
+  const f2_t v = {lo, hi};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, b2_t));
+}
+
 __device__ __host__ inline float bf16_to_f32(bf16_t h) {
   return __builtin_bit_cast(float, (uint32_t)h << 16);
 }

@@ -1301,9 +1301,10 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = f32x4v_t{0.0f, 0.0f, 0.0f, 0.0f};
-    // forward: y of the wave's rows in the tile computed now (tile 0 first)
-    float y_cur = 0.0f;
-    if (!WR) {
+    // y of the wave's rows in the tile computed now (tile 0 first), lane l
+    // holding row l
+    float y_cur;
+    {
       const int64_t n = n_begin + wave * kWsWaveRows + lane;
       y_cur = y[n < N_pad ? n : N_pad - 1];
     }
@@ -1322,14 +1323,13 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
       const char* rp[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) rp[j] = row_ptr(tl, j) + kl * 1024;
-      // forward: y of the wave's 64 rows of the next block's tile (lane l
-      // holding row l, one coalesced load per k block; padded rows read
-      // y = 0).  It is carried into the next iteration (y_cur below), whose
-      // epilogue uses it: a load consumed in its own iteration's epilogue is
-      // placed inside that branch and waited for together with every X
-      // prefetch in flight.
-      float y_next = 0.0f;
-      if (!WR) {
+      // y of the wave's 64 rows of the next block's tile (lane l holding row
+      // l, one coalesced load per k block; padded rows read y = 0).  It is
+      // carried into the next iteration (y_cur below), whose epilogue uses
+      // it: a load consumed in its own iteration's epilogue is placed inside
+      // that branch and waited for together with every X prefetch in flight.
+      float y_next;
+      {
         const int64_t n = n_begin + tl * kWsTile + wave * kWsWaveRows + lane;
         y_next = y[n < N_pad ? n : N_pad - 1];
       }
@@ -1384,7 +1384,7 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
           // K block (its 512 rows span 16 blocks: below 16 * C_pad * 32
           // elements).  Rt holds rows up to N_pad rounded up to a whole tile,
           // so the last tile's stores need no bounds test; rows from N on
-          // get 0.  y is read at min(row, N_pad - 1), in bounds.
+          // get 0.  y comes from the lanes holding the rows (y_cur).
           const int64_t rb = n_begin + t * kWsTile + wave * kWsWaveRows + 4 * (lane >> 4);
           const int64_t row0 = n_begin + t * kWsTile;
           bf16_t* rt = Rt + (row0 >> 5) * ldr * 32;
@@ -1395,7 +1395,7 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
             const int64_t left = n_end - r0;
             float yv[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) yv[v] = y[min(r0 + v, N_pad - 1)];
+            for (int v = 0; v < 4; ++v) yv[v] = __shfl(y_cur, 16 * j + 4 * (lane >> 4) + v, 64);
             const int nl = (int)(r0 - row0);                       // row within the tile
             const uint32_t ro = (uint32_t)(nl >> 5) * (uint32_t)ldr * 32u + (uint32_t)(nl & 31) + c0;
 #pragma unroll
@@ -1415,12 +1415,15 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
                 }
                 const float inv = fast_rcp(1.0f + tt);
                 const float sig = eta >= 0.0f ? inv : tt * inv;
-                rv[v] = v < left ? yv[v] - sig : 0.0f;
+                // rows from n_end on: +0 by a bit mask (no branch around the
+                // residual's arithmetic)
+                const uint32_t keep = v < left ? 0xffffffffu : 0u;
+                rv[v] = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, yv[v] - sig) & keep);
               }
               if (LL) lls[i][0] += (double)s32;
               uint2 pk;
-              pk.x = (uint32_t)f32_to_bf16(rv[0]) | ((uint32_t)f32_to_bf16(rv[1]) << 16);
-              pk.y = (uint32_t)f32_to_bf16(rv[2]) | ((uint32_t)f32_to_bf16(rv[3]) << 16);
+              pk.x = f32x2_to_bf16x2(rv[0], rv[1]);
+              pk.y = f32x2_to_bf16x2(rv[2], rv[3]);
               *(uint2*)(rt + ro + (uint32_t)(16 * i) * 32u) = pk;
             }
           }
@@ -1432,7 +1435,7 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
       }
       t = tn;
       ks0 = kn;
-      if (!WR) y_cur = y_next;
+      y_cur = y_next;
     }
   }
   if (!LL) return;   // uniform across the workgroup: no barrier below is skipped by part of it
