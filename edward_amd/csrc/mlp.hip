@@ -374,13 +374,20 @@ transpose_bf16_kernel(const bf16_t* __restrict__ in, int R, int Cc, bf16_t* __re
 }
 
 // Row sums of a bf16 matrix [R x K] -> fp32 (bias gradients from transposed
-// activation gradients).  One wave per row.
+// activation gradients).  One wave per row, 16-byte loads (K is a multiple
+// of 128: the padded batch).
 __global__ void rowsum_bf16_kernel(const bf16_t* __restrict__ in, int R, int K, float* __restrict__ out) {
   const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (row >= R) return;
   float s = 0.0f;
-  for (int k = lane; k < K; k += 64) s += bf16_to_f32(in[(int64_t)row * K + k]);
+  for (int k = 8 * lane; k < K; k += 512) {
+    const uint4 v = *(const uint4*)(in + (int64_t)row * K + k);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      s += __builtin_bit_cast(float, w[e] << 16) + __builtin_bit_cast(float, w[e] & 0xffff0000u);
+  }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   if (lane == 0) out[row] = s;
