@@ -340,8 +340,9 @@ struct EpiLeapfrog {
 __global__ void norm_in_kernel(const float* __restrict__ W, int B, int D, int64_t ldw,
                                const float* __restrict__ shift, const float* __restrict__ scale,
                                bf16_t* __restrict__ x0, int Bp, int Dp, bf16_t* __restrict__ xcat,
-                               int xcat_ld) {
+                               int xcat_ld, double* __restrict__ zero_out = nullptr) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (zero_out && i == 0) *zero_out = 0.0;   // a loss accumulator the batch's GEMMs add to
   if (i >= (int64_t)Bp * Dp) return;
   const int b = (int)(i / Dp), d = (int)(i - (int64_t)b * Dp);
   float v = 0.0f;
@@ -666,18 +667,18 @@ static int batch_ok(edhmc_ctx* ctx, Surrogate* s, int B) {
   return EDHMC_OK;
 }
 
+// Forward and backward over a batch: the parameter gradient into s->Gr, the
+// mean loss accumulated into *loss (FP64, zeroed by the first kernel).
 static int loss_grad(edhmc_ctx* ctx, Surrogate* s, const float* W, const float* G, int B,
-                     hipStream_t st) {
+                     double* loss, hipStream_t st) {
   const int Bp = (int)round_up(B, 128);
   const float* P = (const float*)s->P.p;
   const float* nrm = (const float*)s->nrm.p;
   hipLaunchKernelGGL(norm_in_kernel, dim3(blocks_for((int64_t)Bp * s->Dp)), dim3(256), 0, st, W,
                      B, s->D, (int64_t)s->D, nrm, nrm + s->Dp, (bf16_t*)s->x0.p, Bp, s->Dp,
-                     (bf16_t*)s->h2.p + s->Hp, s->Kc);
+                     (bf16_t*)s->h2.p + s->Hp, s->Kc, loss);
   EDHMC_TRY(check_launch(ctx, "norm_in"));
   EDHMC_TRY(forward_hidden(ctx, s, Bp, true, st));
-  if (hipMemsetAsync(s->loss.p, 0, 8, st) != hipSuccess)
-    return set_err(ctx, EDHMC_EHIP, "memset loss");
   EpiLoss el;
   el.bias = P + s->ob3;
   el.G = G;
@@ -688,7 +689,7 @@ static int loss_grad(edhmc_ctx* ctx, Surrogate* s, const float* W, const float* 
   el.ld = s->Dp;
   el.coef = 2.0f / ((float)B * (float)s->D);
   el.dO = (bf16_t*)s->dO.p;
-  el.loss = (double*)s->loss.p;
+  el.loss = loss;
   EDHMC_TRY(mlp_gemm(ctx, s->h2.p, s->Kc, s->W3b.p, s->Kc, Bp, s->Dp, s->Kc, el, st));
   float* Gr = (float*)s->Gr.p;
   // layer 3 weights: dW3 = dO^T h2, db3 = column sums of dO
@@ -1136,9 +1137,7 @@ int edhmc_sur_loss_grad(edhmc_ctx* ctx, const float* W, const float* G, int32_t 
   if (!W || !G) return set_err(ctx, EDHMC_EINVAL, "NULL argument");
   hipStream_t st = (hipStream_t)stream;
   s->ghat_valid = false;
-  EDHMC_TRY(loss_grad(ctx, s, W, G, B, st));
-  if (loss_out && hipMemcpyAsync(loss_out, s->loss.p, 8, hipMemcpyDeviceToDevice, st) != hipSuccess)
-    return set_err(ctx, EDHMC_EHIP, "copy loss");
+  EDHMC_TRY(loss_grad(ctx, s, W, G, B, loss_out ? loss_out : (double*)s->loss.p, st));
   if (param_grad_out) {
     hipLaunchKernelGGL(pack_kernel, dim3(blocks_for(s->n_params)), dim3(256), 0, st, layout_of(s),
                        s->n_params, param_grad_out, (float*)s->Gr.p, 0);

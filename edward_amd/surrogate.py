@@ -169,14 +169,24 @@ class Surrogate(object):
     return losses
 
   def validation_loss(self, W, G):
-    """Mean loss over (W, G) in batches of max_batch (no parameter update)."""
-    tot, n = 0.0, 0
-    for s in range(0, int(W.shape[0]), self.max_batch):
-      Wc, Gc = W[s:s + self.max_batch], G[s:s + self.max_batch]
-      loss, _ = self.loss_grad(Wc, Gc)
-      tot += loss.item() * int(Wc.shape[0])
-      n += int(Wc.shape[0])
-    return tot / max(n, 1)
+    """Mean loss over (W, G) in batches of max_batch (no parameter update;
+    the batches' losses land in one device array, read once)."""
+    eng = self.engine
+    W = W.to(eng.device, torch.float32).contiguous()
+    G = G.to(eng.device, torch.float32).contiguous()
+    starts = list(range(0, int(W.shape[0]), self.max_batch))
+    losses = torch.empty(max(len(starts), 1), dtype=torch.float64, device=eng.device)
+    sizes = []
+    for k, s in enumerate(starts):
+      n = min(self.max_batch, int(W.shape[0]) - s)
+      eng.check(eng.lib.edhmc_sur_loss_grad(
+          eng.h, L.ptr(W[s:s + n]), L.ptr(G[s:s + n]), n, L.ptr(losses[k:k + 1]), None,
+          eng.stream))
+      sizes.append(n)
+    if not sizes:
+      return 0.0
+    ls = losses.cpu().numpy()[:len(sizes)]
+    return float(np.dot(ls, np.asarray(sizes, np.float64)) / sum(sizes))
 
   def fit(self, W, G, epochs=20, batch=4096, lr=1e-3, lr_final=1e-4, seed=0, normalise=True,
           log=None, val_frac=0.0):
@@ -214,10 +224,15 @@ class Surrogate(object):
     decay = (lr_final / lr) ** (1.0 / total) if lr_final else 1.0
     losses = []
     step = 0
+    perm = torch.as_tensor(train_rows[rng.permutation(P)], dtype=torch.int64).to(dev)
     for ep in range(epochs):
-      perm = torch.as_tensor(train_rows[rng.permutation(P)], dtype=torch.int64).to(dev)
       lrs = [lr * decay ** (step + b) for b in range(n_batches)]
-      ep_loss = float(self.train_epoch(W, G, perm, batch, lrs).cpu().numpy().mean())
+      ep_losses = self.train_epoch(W, G, perm, batch, lrs)
+      # the next epoch's shuffle is drawn while this one runs on the device
+      # (the copy waits for the epoch's use of the current one: same stream)
+      if ep + 1 < epochs:
+        perm = torch.as_tensor(train_rows[rng.permutation(P)], dtype=torch.int64).to(dev)
+      ep_loss = float(ep_losses.cpu().numpy().mean())
       step += n_batches
       losses.append(ep_loss)
       msg = 'surrogate epoch %d loss %.5f' % (ep, ep_loss)
