@@ -19,6 +19,7 @@
 #include "common.h"
 #include "ctx.h"
 #include "gemm_nt.h"
+#include "gemm2_dma.h"
 #include "philox.h"
 #include "transform.h"
 
@@ -80,6 +81,11 @@ using Cfg1 = TileCfg<EDHMC_CFG_G1>;    // GEMM1 (gradient pass, one plane)
 // each), four waves per SIMD (4.33 -> 3.73 ms at N = 1e6, D = 512, 4096
 // chains against 4 waves of 128 x 128, which hold one wave per SIMD)
 using CfgG2 = TileCfg<EDHMC_CFG_G2>;
+// bf16 GEMM2 with LDS-direct stage loads (gemm2_dma.h) where it applies
+#ifndef EDHMC_G2_DMA
+#define EDHMC_G2_DMA 1
+#endif
+constexpr bool kG2Dma = EDHMC_G2_DMA != 0;
 using Cfg2S = TileCfg<1, 1, 4, 1>;     // 128 x 32, GEMM2 for narrow D
 using CfgP2 = TileCfg<EDHMC_CFG_P2>;   // GEMM1 over hi/lo split positions
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
@@ -1097,8 +1103,42 @@ static int launch_gemm2_cfg(edhmc_ctx* ctx, hipStream_t st) {
   return check_launch(ctx, "gemm2");
 }
 
+// bf16 GEMM2 on 256 x 256 tiles with stages loaded straight into LDS
+// (gemm2_dma.h); same split-K and tile order as launch_gemm2_cfg, same bits.
+static int launch_gemm2_dma(edhmc_ctx* ctx, hipStream_t st) {
+  GemmShape g = {};
+  g.A = (const char*)ctx->Rt.p;
+  g.lda_b = kStageBytes;
+  g.a_stage_b = (int64_t)ctx->C_pad * kStageBytes;
+  g.B = (const char*)ctx->Xc.p;
+  g.ldb_b = kStageBytes;
+  g.b_stage_b = (int64_t)ctx->D_pad * kStageBytes;
+  g.m_tiles = ctx->C_pad / kG2Tile;
+  g.n_tiles = ctx->D_pad / kG2Tile;
+  g.k_stages = (int)(ctx->N_pad * 2 / kStageBytes);
+  g.ksplit = ctx->ksplit;
+  g.stages_per_split = (g.k_stages + g.ksplit - 1) / g.ksplit;
+  g.swz = (g.ksplit % 8 == 0) ? 2 : 0;
+  ProfScope ps(ctx, PK_GEMM_XTR, st);
+  hipLaunchKernelGGL(gemm2_dma_kernel, dim3(grid_size(g)), dim3(256), 0, st, g,
+                     (float*)ctx->Gpart.p, (int64_t)ctx->D_pad,
+                     (int64_t)ctx->C_pad * ctx->D_pad);
+  return check_launch(ctx, "gemm2");
+}
+
+// Whether launch_gemm2_dma applies: bf16, whole 256 x 256 tiles, and every
+// K split at least kG2Ring stages (the kernel's prologue fills its ring).
+static bool gemm2_dma_ok(const edhmc_ctx* ctx) {
+  if (!kG2Dma || ctx->cdt != EDHMC_BF16 || ctx->D_pad % kG2Tile || ctx->C_pad % kG2Tile)
+    return false;
+  const int64_t kst = ctx->N_pad * 2 / kStageBytes;
+  const int64_t spl = (kst + ctx->ksplit - 1) / ctx->ksplit;
+  return spl >= kG2Ring && kst - (int64_t)(ctx->ksplit - 1) * spl >= kG2Ring;
+}
+
 template <typename T>
 static int launch_gemm2(edhmc_ctx* ctx, hipStream_t st) {
+  if (std::is_same<T, bf16_t>::value && gemm2_dma_ok(ctx)) return launch_gemm2_dma(ctx, st);
   if (ctx->D_pad % CfgG2::BN == 0 && ctx->C_pad % CfgG2::BM == 0)
     return launch_gemm2_cfg<T, CfgG2>(ctx, st);
   return launch_gemm2_cfg<T, Cfg2S>(ctx, st);

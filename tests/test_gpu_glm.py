@@ -245,6 +245,33 @@ def test_run_semantics_and_moments():
   assert n_acc.sum().item() > 0
 
 
+@pytest.mark.parametrize('N,D,C', [(65536, 512, 512), (100000, 256, 256)])
+def test_bf16_gradient_on_the_direct_load_gemm2(N, D, C):
+  """bf16 gradients at shapes whose X^T R product runs on the 256 x 256
+  direct-to-LDS kernel (whole tiles, every K split >= 5 stages): 64 splits
+  of 32 stages (a whole number of five-stage groups plus a partial one),
+  and 64 splits of 49 stages with a last split of 41 (stages past a split's
+  end read as zeros), against the oracle's gradient with the pass's own
+  roundings (GLM.grad_bf16, two position planes, bf16 residuals, FP64
+  sums): max error within 1e-4 of max|grad| (measured 2.7e-5 and 1.4e-5:
+  fp32 sums over up to 100,000 rows, and residuals whose fp32 eta rounds to
+  the other side of a bf16 tie), two evaluations bitwise equal."""
+  from edward_amd import workloads as W
+  print('case N=%d D=%d C=%d' % (N, D, C), flush=True)
+  wl = W.logreg(N, D, C, 0.002, 8, 10, seed=41, bf16=True)
+  eng = _engine(wl, precision='bf16')
+  rng = np.random.default_rng(9)
+  W0 = (wl.w_true[None, :] + 0.01 * rng.normal(size=(C, D))).astype(np.float32)
+  _, g1 = eng.logp_grad(W0)
+  _, g2 = eng.logp_grad(W0)
+  g1, g2 = g1.cpu().numpy(), g2.cpu().numpy()
+  np.testing.assert_array_equal(g1, g2)
+  g_ref = _oracle(wl).grad_bf16(W0.astype(np.float64), planes=2)
+  err = np.abs(g1 - g_ref).max() / np.abs(g_ref).max()
+  print('measured: max gradient error %.2e of max|grad|' % err, flush=True)
+  assert err < 1e-4, err
+
+
 def test_mid_workload_full_size_logp_grad():
   """N=1e5, D=256 (column-scaled X), all 1024 chains on the GPU; 12 of them
   checked against the FP64 oracle."""
