@@ -79,9 +79,8 @@ struct Surrogate {
   // reads buffer k & 1 while its epilogue writes the next input into the
   // other one, so no GEMM operand changes while the GEMM runs
   DevBuf h2x;
-  DevBuf dO, da2, da1, x0t, h1t, h2t, dOt, da2t, da1t;
+  DevBuf dO, da2, da1, h1t, h2t, dOt, da2t, da1t;
   DevBuf loss;                         // fp64 [1]
-  DevBuf Wb, Gb;                       // a gathered training batch [Bmax x D] fp32 each
   // sampling state
   DevBuf ghat, ghat0;                  // [C x D_pad of the GLM]
   bool ghat_valid = false;
@@ -96,8 +95,8 @@ void surrogate_destroy(Surrogate* s) {
   if (!s) return;
   DevBuf* b[] = {&s->P, &s->Gr, &s->Am, &s->Av, &s->W1b, &s->W2b, &s->W3b, &s->W2t, &s->W3t,
                  &s->nrm, &s->x0, &s->a1, &s->h1, &s->a2, &s->h2, &s->h2x, &s->o, &s->dO, &s->da2,
-                 &s->da1, &s->x0t, &s->h1t, &s->h2t, &s->dOt, &s->da2t, &s->da1t, &s->loss,
-                 &s->Wb, &s->Gb, &s->ghat, &s->ghat0};
+                 &s->da1, &s->h1t, &s->h2t, &s->dOt, &s->da2t, &s->da1t, &s->loss,
+                 &s->ghat, &s->ghat0};
   for (DevBuf* x : b) x->release();
   delete s;
 }
@@ -106,11 +105,23 @@ void surrogate_destroy(Surrogate* s) {
 // Epilogues
 // ---------------------------------------------------------------------------
 // a = acc + bias; store a (fp32, optional) and h = GELU(a) (bf16).
+// Four bf16 values of rows m .. m + 3 (m a multiple of 4) in column n of a
+// matrix, stored into its transpose out [N x ldt]: one 8-byte store.  In a
+// 32 x 32 accumulator block, elements 4q .. 4q + 3 of a lane are such rows.
+__device__ inline void store_t4(bf16_t* out, int64_t ldt, int n, int64_t m, const bf16_t (&v)[4]) {
+  uint2 pk;
+  pk.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
+  pk.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
+  *(uint2*)(out + (int64_t)n * ldt + m) = pk;
+}
+
 struct EpiBiasGelu {
   const float* bias;
   float* a_out;      // [M x ld] or null
   bf16_t* h_out;     // [M x ldh]
   int64_t ld, ldh;
+  bf16_t* ht_out = nullptr;   // h transposed [N x ldt] (training), or null
+  int64_t ldt = 0;
   template <typename Cfg>
   __device__ void apply(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p) const {
 #pragma unroll
@@ -120,21 +131,30 @@ struct EpiBiasGelu {
 #pragma unroll
       for (int i = 0; i < Cfg::kWM; ++i)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int64_t m = p.m0 + 32 * i + acc_row(r, p.lane);
-          const float a = acc[i][j][r] + bj;
-          if (a_out) a_out[m * ld + n] = a;
-          h_out[m * ldh + n] = f32_to_bf16(gelu(a));
+        for (int q = 0; q < 4; ++q) {
+          bf16_t hb[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int r = 4 * q + v;
+            const int64_t m = p.m0 + 32 * i + acc_row(r, p.lane);
+            const float a = acc[i][j][r] + bj;
+            if (a_out) a_out[m * ld + n] = a;
+            hb[v] = f32_to_bf16(gelu(a));
+            h_out[m * ldh + n] = hb[v];
+          }
+          if (ht_out) store_t4(ht_out, ldt, n, p.m0 + 32 * i + acc_row(4 * q, p.lane), hb);
         }
     }
   }
 };
 
-// da = acc * GELU'(a) (bf16).
+// da = acc * GELU'(a) (bf16); optionally also transposed.
 struct EpiGeluBack {
   const float* a_in;
   bf16_t* da_out;
   int64_t ld;
+  bf16_t* dat_out = nullptr;   // da transposed [N x ldt], or null
+  int64_t ldt = 0;
   template <typename Cfg>
   __device__ void apply(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p) const {
 #pragma unroll
@@ -143,23 +163,36 @@ struct EpiGeluBack {
 #pragma unroll
       for (int i = 0; i < Cfg::kWM; ++i)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int64_t m = p.m0 + 32 * i + acc_row(r, p.lane);
-          da_out[m * ld + n] = f32_to_bf16(acc[i][j][r] * gelu_grad(a_in[m * ld + n]));
+        for (int q = 0; q < 4; ++q) {
+          bf16_t db[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int r = 4 * q + v;
+            const int64_t m = p.m0 + 32 * i + acc_row(r, p.lane);
+            db[v] = f32_to_bf16(acc[i][j][r] * gelu_grad(a_in[m * ld + n]));
+            da_out[m * ld + n] = db[v];
+          }
+          if (dat_out) store_t4(dat_out, ldt, n, p.m0 + 32 * i + acc_row(4 * q, p.lane), db);
         }
     }
   }
 };
 
-// Training output: o = acc + b3; loss += (o - t)^2; dO = 2 (o - t) / (B D).
+// Training output: o = acc + b3; loss += (o - t)^2; dO = 2 (o - t) / (B D),
+// also transposed.  Target row m is G[idx[m]] when idx is given (a row
+// index outside [0, P) reads NaN, so the batch's loss is NaN), else G[m].
 struct EpiLoss {
   const float* bias;
-  const float* G;            // targets [B x D] (raw gradients)
+  const float* G;            // targets [B x D] (raw gradients), or [P x D] with idx
+  const int64_t* idx;        // batch rows of G, or null
+  int64_t P;
   const float* out_shift; const float* out_scale;
   int B, D;
   int64_t ld;                // Dp
   float coef;                // 2 / (B D)
   bf16_t* dO;
+  bf16_t* dOt;               // dO transposed [Dp x ldt]
+  int64_t ldt;
   double* loss;              // fp64 accumulator
   template <typename Cfg>
   __device__ void apply(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p) const {
@@ -173,15 +206,28 @@ struct EpiLoss {
 #pragma unroll
       for (int i = 0; i < Cfg::kWM; ++i)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int m = p.m0 + 32 * i + acc_row(r, p.lane);
-          float d = 0.0f;
-          if (nv && m < B) {
-            const float t = (G[(int64_t)m * D + n] - sh) * isc;
-            d = acc[i][j][r] + bj - t;
-            part += (double)d * (double)d;
+        for (int q = 0; q < 4; ++q) {
+          bf16_t ob[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int m = p.m0 + 32 * i + acc_row(4 * q + v, p.lane);
+            float d = 0.0f;
+            if (nv && m < B) {
+              float g;
+              if (idx) {
+                const int64_t row = idx[m];
+                g = (row >= 0 && row < P) ? G[row * D + n] : __builtin_nanf("");
+              } else {
+                g = G[(int64_t)m * D + n];
+              }
+              const float t = (g - sh) * isc;
+              d = acc[i][j][4 * q + v] + bj - t;
+              part += (double)d * (double)d;
+            }
+            ob[v] = f32_to_bf16(coef * d);
+            dO[(int64_t)m * ld + n] = ob[v];
           }
-          dO[(int64_t)m * ld + n] = f32_to_bf16(coef * d);
+          store_t4(dOt, ldt, n, p.m0 + 32 * i + acc_row(4 * q, p.lane), ob);
         }
     }
     part = wave_sum_f64(part) * (0.5 * (double)coef);   // mean over B x D
@@ -340,13 +386,24 @@ struct EpiLeapfrog {
 __global__ void norm_in_kernel(const float* __restrict__ W, int B, int D, int64_t ldw,
                                const float* __restrict__ shift, const float* __restrict__ scale,
                                bf16_t* __restrict__ x0, int Bp, int Dp, bf16_t* __restrict__ xcat,
-                               int xcat_ld, double* __restrict__ zero_out = nullptr) {
+                               int xcat_ld, double* __restrict__ zero_out = nullptr,
+                               const int64_t* __restrict__ idx = nullptr, int64_t P = 0) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (zero_out && i == 0) *zero_out = 0.0;   // a loss accumulator the batch's GEMMs add to
   if (i >= (int64_t)Bp * Dp) return;
   const int b = (int)(i / Dp), d = (int)(i - (int64_t)b * Dp);
   float v = 0.0f;
-  if (b < B && d < D) v = (W[(int64_t)b * ldw + d] - shift[d]) * scale[d];
+  if (b < B && d < D) {
+    // row b of the batch: W[idx[b]] of the P rows (NaN outside [0, P)), or W[b]
+    float w;
+    if (idx) {
+      const int64_t r = idx[b];
+      w = (r >= 0 && r < P) ? W[r * ldw + d] : __builtin_nanf("");
+    } else {
+      w = W[(int64_t)b * ldw + d];
+    }
+    v = (w - shift[d]) * scale[d];
+  }
   const bf16_t xb = f32_to_bf16(v);
   x0[i] = xb;
   xcat[(int64_t)b * xcat_ld + d] = xb;
@@ -363,15 +420,24 @@ __global__ void gather_rows_kernel(const float* __restrict__ src, int64_t P, int
   dst[i] = (r >= 0 && r < P) ? src[r * D + d] : __builtin_nanf("");
 }
 
-// out [C x R] = in [R x C]^T for bf16, 64 x 64 tiles (R, C multiples of 64).
+// out [C x R] = in [R x C]^T for bf16, 64 x 64 tiles (R, C multiples of 64);
+// blockIdx.z picks one of two such transposes (tiles outside a matrix idle).
+struct TransposeJob {
+  const bf16_t* in;
+  int R, Cc;
+  bf16_t* out;
+};
+
 __global__ void __launch_bounds__(256)
-transpose_bf16_kernel(const bf16_t* __restrict__ in, int R, int Cc, bf16_t* __restrict__ out) {
+transpose_bf16_kernel(TransposeJob j0, TransposeJob j1) {
+  const TransposeJob& jb = blockIdx.z ? j1 : j0;
   __shared__ bf16_t t[64][66];
   const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  if (r0 >= jb.R || c0 >= jb.Cc) return;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  for (int k = ty; k < 64; k += 4) t[k][tx] = in[(int64_t)(r0 + k) * Cc + c0 + tx];
+  for (int k = ty; k < 64; k += 4) t[k][tx] = jb.in[(int64_t)(r0 + k) * jb.Cc + c0 + tx];
   __syncthreads();
-  for (int k = ty; k < 64; k += 4) out[(int64_t)(c0 + k) * R + r0 + tx] = t[tx][k];
+  for (int k = ty; k < 64; k += 4) jb.out[(int64_t)(c0 + k) * jb.R + r0 + tx] = t[tx][k];
 }
 
 // Row sums of a bf16 matrix [R x K] -> fp32 (bias gradients from transposed
@@ -498,10 +564,17 @@ affine_rows_kernel(const float* __restrict__ E, int64_t n, int D, const float* _
     }
 }
 
-// Adam (Kingma & Ba 2015) over the padded parameter vector.
+// Adam (Kingma & Ba 2015) over the padded parameter vector.  With w1b..w3b
+// given, the updated weights of the three layers are also written as the
+// bf16 copies the GEMMs read (elements [oW_l, oW_l + n_l) of P).
+struct AdamBf16Out {
+  bf16_t* w[3];
+  int64_t off[3], n[3];
+};
+
 __global__ void adam_kernel(float* __restrict__ P, const float* __restrict__ G, float* __restrict__ M,
                             float* __restrict__ V, int64_t n, float lr, float b1, float b2,
-                            float eps, float c1, float c2) {
+                            float eps, float c1, float c2, AdamBf16Out wb) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float g = G[i];
@@ -509,7 +582,11 @@ __global__ void adam_kernel(float* __restrict__ P, const float* __restrict__ G, 
   const float v = b2 * V[i] + (1.0f - b2) * g * g;
   M[i] = m;
   V[i] = v;
-  P[i] -= lr * (m * c1) / (sqrtf(v * c2) + eps);
+  const float p = P[i] - lr * (m * c1) / (sqrtf(v * c2) + eps);
+  P[i] = p;
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+    if (wb.w[l] && i >= wb.off[l] && i < wb.off[l] + wb.n[l]) wb.w[l][i - wb.off[l]] = f32_to_bf16(p);
 }
 
 // fp32 [R x C] -> bf16 [R x C]
@@ -605,10 +682,19 @@ static int mlp_gemm(edhmc_ctx* ctx, const void* A, int64_t lda_elems, const void
   return mlp_gemm_cfg<CfgMS>(ctx, A, lda_elems, B, ldb_elems, M, N, K, e, st);
 }
 
-static int transpose(edhmc_ctx* ctx, const DevBuf& in, int R, int Cc, DevBuf& out, hipStream_t st) {
-  hipLaunchKernelGGL(transpose_bf16_kernel, dim3(Cc / 64, R / 64), dim3(256), 0, st,
-                     (const bf16_t*)in.p, R, Cc, (bf16_t*)out.p);
+// One launch for one or two bf16 transposes (in [R x Cc] -> out [Cc x R]).
+static int transpose(edhmc_ctx* ctx, TransposeJob a, const TransposeJob* b, hipStream_t st) {
+  const TransposeJob bb = b ? *b : TransposeJob{a.in, 0, 0, a.out};
+  const int gx = std::max(a.Cc, bb.Cc) / 64, gy = std::max(a.R, bb.R) / 64;
+  hipLaunchKernelGGL(transpose_bf16_kernel, dim3(gx, gy, b ? 2 : 1), dim3(256), 0, st, a, bb);
   return check_launch(ctx, "transpose");
+}
+
+// The transposed bf16 weights the backward GEMMs read, from the forward copies.
+static int transpose_weights(edhmc_ctx* ctx, Surrogate* s, hipStream_t st) {
+  const TransposeJob w2{(const bf16_t*)s->W2b.p, s->Hp, s->Hp, (bf16_t*)s->W2t.p};
+  const TransposeJob w3{(const bf16_t*)s->W3b.p, s->Dp, s->Kc, (bf16_t*)s->W3t.p};
+  return transpose(ctx, w2, &w3, st);
 }
 
 // Refresh the bf16 weight copies (forward and transposed) from the masters.
@@ -622,8 +708,7 @@ static int refresh_weights(edhmc_ctx* ctx, Surrogate* s, hipStream_t st) {
   EDHMC_TRY(cvt(s->oW1, (int64_t)s->Hp * s->Dp, s->W1b));
   EDHMC_TRY(cvt(s->oW2, (int64_t)s->Hp * s->Hp, s->W2b));
   EDHMC_TRY(cvt(s->oW3, (int64_t)s->Dp * s->Kc, s->W3b));
-  EDHMC_TRY(transpose(ctx, s->W2b, s->Hp, s->Hp, s->W2t, st));
-  EDHMC_TRY(transpose(ctx, s->W3b, s->Dp, s->Kc, s->W3t, st));
+  EDHMC_TRY(transpose_weights(ctx, s, st));
   s->ghat_valid = false;
   return EDHMC_OK;
 }
@@ -633,7 +718,8 @@ static int refresh_weights(edhmc_ctx* ctx, Surrogate* s, hipStream_t st) {
 // pass; h2 goes into the first Hp columns of xcat ([Bp x Kc]; s->h2 unless
 // given).
 static int forward_hidden(edhmc_ctx* ctx, Surrogate* s, int Bp, bool keep_a, hipStream_t st,
-                          bf16_t* xcat = nullptr, bool x_from_xcat = false) {
+                          bf16_t* xcat = nullptr, bool x_from_xcat = false,
+                          bf16_t* h1t = nullptr, bf16_t* h2t = nullptr) {
   const float* P = (const float*)s->P.p;
   EpiBiasGelu e1;
   e1.bias = P + s->ob1;
@@ -641,6 +727,8 @@ static int forward_hidden(edhmc_ctx* ctx, Surrogate* s, int Bp, bool keep_a, hip
   e1.h_out = (bf16_t*)s->h1.p;
   e1.ld = s->Hp;
   e1.ldh = s->Hp;
+  e1.ht_out = h1t;
+  e1.ldt = Bp;
   bf16_t* xc = xcat ? xcat : (bf16_t*)s->h2.p;
   if (x_from_xcat)
     EDHMC_TRY(mlp_gemm(ctx, xc + s->Hp, s->Kc, s->W1b.p, s->Dp, Bp, s->Hp, s->Dp, e1, st));
@@ -652,6 +740,8 @@ static int forward_hidden(edhmc_ctx* ctx, Surrogate* s, int Bp, bool keep_a, hip
   e2.h_out = xc;
   e2.ld = s->Hp;
   e2.ldh = s->Kc;
+  e2.ht_out = h2t;
+  e2.ldt = Bp;
   return mlp_gemm(ctx, s->h1.p, s->Hp, s->W2b.p, s->Hp, Bp, s->Hp, s->Hp, e2, st);
 }
 
@@ -668,20 +758,29 @@ static int batch_ok(edhmc_ctx* ctx, Surrogate* s, int B) {
 }
 
 // Forward and backward over a batch: the parameter gradient into s->Gr, the
-// mean loss accumulated into *loss (FP64, zeroed by the first kernel).
+// mean loss accumulated into *loss (FP64, zeroed by the first kernel).  The
+// batch is rows idx[0..B) of W and G ([P x D]) when idx is given, else their
+// first B rows.  The forward and activation-gradient epilogues also write
+// the transposed activations the weight-gradient GEMMs read (their K is the
+// batch); h2t holds [h2 | x0]^T, its last Dp rows being x0^T.
 static int loss_grad(edhmc_ctx* ctx, Surrogate* s, const float* W, const float* G, int B,
-                     double* loss, hipStream_t st) {
+                     double* loss, hipStream_t st, const int64_t* idx = nullptr, int64_t P_rows = 0) {
   const int Bp = (int)round_up(B, 128);
   const float* P = (const float*)s->P.p;
   const float* nrm = (const float*)s->nrm.p;
+  bf16_t* h2t = (bf16_t*)s->h2t.p;
+  bf16_t* x0t = h2t + (int64_t)s->Hp * Bp;
   hipLaunchKernelGGL(norm_in_kernel, dim3(blocks_for((int64_t)Bp * s->Dp)), dim3(256), 0, st, W,
                      B, s->D, (int64_t)s->D, nrm, nrm + s->Dp, (bf16_t*)s->x0.p, Bp, s->Dp,
-                     (bf16_t*)s->h2.p + s->Hp, s->Kc, loss);
+                     (bf16_t*)s->h2.p + s->Hp, s->Kc, loss, idx, P_rows);
   EDHMC_TRY(check_launch(ctx, "norm_in"));
-  EDHMC_TRY(forward_hidden(ctx, s, Bp, true, st));
+  EDHMC_TRY(transpose(ctx, TransposeJob{(const bf16_t*)s->x0.p, Bp, s->Dp, x0t}, nullptr, st));
+  EDHMC_TRY(forward_hidden(ctx, s, Bp, true, st, nullptr, false, (bf16_t*)s->h1t.p, h2t));
   EpiLoss el;
   el.bias = P + s->ob3;
   el.G = G;
+  el.idx = idx;
+  el.P = P_rows;
   el.out_shift = nrm + 2 * s->Dp;
   el.out_scale = nrm + 3 * s->Dp;
   el.B = B;
@@ -689,17 +788,17 @@ static int loss_grad(edhmc_ctx* ctx, Surrogate* s, const float* W, const float* 
   el.ld = s->Dp;
   el.coef = 2.0f / ((float)B * (float)s->D);
   el.dO = (bf16_t*)s->dO.p;
+  el.dOt = (bf16_t*)s->dOt.p;
+  el.ldt = Bp;
   el.loss = loss;
   EDHMC_TRY(mlp_gemm(ctx, s->h2.p, s->Kc, s->W3b.p, s->Kc, Bp, s->Dp, s->Kc, el, st));
   float* Gr = (float*)s->Gr.p;
-  // layer 3 weights: dW3 = dO^T h2, db3 = column sums of dO
-  EDHMC_TRY(transpose(ctx, s->dO, Bp, s->Dp, s->dOt, st));
-  EDHMC_TRY(transpose(ctx, s->h2, Bp, s->Kc, s->h2t, st));
+  // layer 3 weights: dW3 = dO^T [h2 | x0], db3 = column sums of dO
   EpiStoreF32 ew;
   ew.split_b = 0;
   ew.out = Gr + s->oW3;
   ew.ldc = s->Kc;
-  EDHMC_TRY(mlp_gemm(ctx, s->dOt.p, Bp, s->h2t.p, Bp, s->Dp, s->Kc, Bp, ew, st));
+  EDHMC_TRY(mlp_gemm(ctx, s->dOt.p, Bp, h2t, Bp, s->Dp, s->Kc, Bp, ew, st));
   hipLaunchKernelGGL(rowsum_bf16_kernel, dim3(blocks_for(s->Dp, 4)), dim3(256), 0, st,
                      (const bf16_t*)s->dOt.p, s->Dp, Bp, Gr + s->ob3);
   EDHMC_TRY(check_launch(ctx, "rowsum"));
@@ -708,9 +807,9 @@ static int loss_grad(edhmc_ctx* ctx, Surrogate* s, const float* W, const float* 
   eb2.a_in = (const float*)s->a2.p;
   eb2.da_out = (bf16_t*)s->da2.p;
   eb2.ld = s->Hp;
+  eb2.dat_out = (bf16_t*)s->da2t.p;
+  eb2.ldt = Bp;
   EDHMC_TRY(mlp_gemm(ctx, s->dO.p, s->Dp, s->W3t.p, s->Dp, Bp, s->Hp, s->Dp, eb2, st));
-  EDHMC_TRY(transpose(ctx, s->da2, Bp, s->Hp, s->da2t, st));
-  EDHMC_TRY(transpose(ctx, s->h1, Bp, s->Hp, s->h1t, st));
   ew.out = Gr + s->oW2;
   ew.ldc = s->Hp;
   EDHMC_TRY(mlp_gemm(ctx, s->da2t.p, Bp, s->h1t.p, Bp, s->Hp, s->Hp, Bp, ew, st));
@@ -722,12 +821,12 @@ static int loss_grad(edhmc_ctx* ctx, Surrogate* s, const float* W, const float* 
   eb1.a_in = (const float*)s->a1.p;
   eb1.da_out = (bf16_t*)s->da1.p;
   eb1.ld = s->Hp;
+  eb1.dat_out = (bf16_t*)s->da1t.p;
+  eb1.ldt = Bp;
   EDHMC_TRY(mlp_gemm(ctx, s->da2.p, s->Hp, s->W2t.p, s->Hp, Bp, s->Hp, s->Hp, eb1, st));
-  EDHMC_TRY(transpose(ctx, s->da1, Bp, s->Hp, s->da1t, st));
-  EDHMC_TRY(transpose(ctx, s->x0, Bp, s->Dp, s->x0t, st));
   ew.out = Gr + s->oW1;
   ew.ldc = s->Dp;
-  EDHMC_TRY(mlp_gemm(ctx, s->da1t.p, Bp, s->x0t.p, Bp, s->Hp, s->Dp, Bp, ew, st));
+  EDHMC_TRY(mlp_gemm(ctx, s->da1t.p, Bp, x0t, Bp, s->Hp, s->Dp, Bp, ew, st));
   hipLaunchKernelGGL(rowsum_bf16_kernel, dim3(blocks_for(s->Hp, 4)), dim3(256), 0, st,
                      (const bf16_t*)s->da1t.p, s->Hp, Bp, Gr + s->ob1);
   return check_launch(ctx, "rowsum");
@@ -1034,7 +1133,6 @@ int edhmc_sur_create(edhmc_ctx* ctx, int32_t D, int32_t H, int32_t max_batch) {
   EDHMC_TRY(dev_alloc(ctx, s->dO, bd * 2));
   EDHMC_TRY(dev_alloc(ctx, s->da2, bh * 2));
   EDHMC_TRY(dev_alloc(ctx, s->da1, bh * 2));
-  EDHMC_TRY(dev_alloc(ctx, s->x0t, bd * 2));
   EDHMC_TRY(dev_alloc(ctx, s->h1t, bh * 2));
   EDHMC_TRY(dev_alloc(ctx, s->h2t, (size_t)s->Bp * s->Kc * 2));
   EDHMC_TRY(dev_alloc(ctx, s->dOt, bd * 2));
@@ -1146,6 +1244,27 @@ int edhmc_sur_loss_grad(edhmc_ctx* ctx, const float* W, const float* G, int32_t 
   return EDHMC_OK;
 }
 
+// One Adam step on a batch (rows idx[0..B) of the P rows of W and G, or the
+// first B rows): loss and parameter gradient, the update, whose kernel also
+// writes the layers' bf16 weight copies, and the transposed copies.
+static int train_step(edhmc_ctx* ctx, Surrogate* s, const float* W, const float* G, int B,
+                      float lr, float beta1, float beta2, float adam_eps, int step,
+                      double* loss_out, hipStream_t st, const int64_t* idx, int64_t P_rows) {
+  s->ghat_valid = false;
+  EDHMC_TRY(loss_grad(ctx, s, W, G, B, loss_out ? loss_out : (double*)s->loss.p, st, idx, P_rows));
+  const float c1 = (float)(1.0 / (1.0 - std::pow((double)beta1, step)));
+  const float c2 = (float)(1.0 / (1.0 - std::pow((double)beta2, step)));
+  AdamBf16Out wb;
+  wb.w[0] = (bf16_t*)s->W1b.p; wb.off[0] = s->oW1; wb.n[0] = (int64_t)s->Hp * s->Dp;
+  wb.w[1] = (bf16_t*)s->W2b.p; wb.off[1] = s->oW2; wb.n[1] = (int64_t)s->Hp * s->Hp;
+  wb.w[2] = (bf16_t*)s->W3b.p; wb.off[2] = s->oW3; wb.n[2] = (int64_t)s->Dp * s->Kc;
+  hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(s->n_pad)), dim3(256), 0, st, (float*)s->P.p,
+                     (const float*)s->Gr.p, (float*)s->Am.p, (float*)s->Av.p, s->n_pad, lr, beta1,
+                     beta2, adam_eps, c1, c2, wb);
+  EDHMC_TRY(check_launch(ctx, "adam"));
+  return transpose_weights(ctx, s, st);
+}
+
 int edhmc_sur_train_step(edhmc_ctx* ctx, const float* W, const float* G, int32_t B, float lr,
                          float beta1, float beta2, float adam_eps, int32_t step, double* loss_out,
                          void* stream) {
@@ -1154,15 +1273,10 @@ int edhmc_sur_train_step(edhmc_ctx* ctx, const float* W, const float* G, int32_t
   EDHMC_TRY(check_sur(ctx));
   Surrogate* s = ctx->sur;
   if (step < 1) return set_err(ctx, EDHMC_EINVAL, "Adam step count starts at 1");
-  EDHMC_TRY(edhmc_sur_loss_grad(ctx, W, G, B, loss_out, nullptr, stream));
-  hipStream_t st = (hipStream_t)stream;
-  const float c1 = (float)(1.0 / (1.0 - std::pow((double)beta1, step)));
-  const float c2 = (float)(1.0 / (1.0 - std::pow((double)beta2, step)));
-  hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(s->n_pad)), dim3(256), 0, st, (float*)s->P.p,
-                     (const float*)s->Gr.p, (float*)s->Am.p, (float*)s->Av.p, s->n_pad, lr, beta1,
-                     beta2, adam_eps, c1, c2);
-  EDHMC_TRY(check_launch(ctx, "adam"));
-  return refresh_weights(ctx, s, st);
+  EDHMC_TRY(batch_ok(ctx, s, B));
+  if (!W || !G) return set_err(ctx, EDHMC_EINVAL, "NULL argument");
+  return train_step(ctx, s, W, G, B, lr, beta1, beta2, adam_eps, step, loss_out,
+                    (hipStream_t)stream, nullptr, 0);
 }
 
 int edhmc_sur_train_epoch(edhmc_ctx* ctx, const float* W, const float* G, int64_t P,
@@ -1180,22 +1294,11 @@ int edhmc_sur_train_epoch(edhmc_ctx* ctx, const float* W, const float* G, int64_
     return set_err(ctx, EDHMC_EINVAL, "%d batches of %d exceed the %lld pairs", n_batches, batch,
                    (long long)P);
   if (step0 < 0) return set_err(ctx, EDHMC_EINVAL, "Adam step count must be >= 0");
-  const size_t bytes = (size_t)s->Bmax * s->D * 4;
-  if (s->Wb.bytes != bytes) EDHMC_TRY(dev_alloc(ctx, s->Wb, bytes, false));
-  if (s->Gb.bytes != bytes) EDHMC_TRY(dev_alloc(ctx, s->Gb, bytes, false));
-  hipStream_t st = (hipStream_t)stream;
-  const unsigned gb = blocks_for((int64_t)batch * s->D);
-  for (int b = 0; b < n_batches; ++b) {
-    const int64_t* idx = perm + (int64_t)b * batch;
-    hipLaunchKernelGGL(gather_rows_kernel, dim3(gb), dim3(256), 0, st, W, P, s->D, idx, batch,
-                       (float*)s->Wb.p);
-    hipLaunchKernelGGL(gather_rows_kernel, dim3(gb), dim3(256), 0, st, G, P, s->D, idx, batch,
-                       (float*)s->Gb.p);
-    EDHMC_TRY(check_launch(ctx, "gather_rows"));
-    EDHMC_TRY(edhmc_sur_train_step(ctx, (const float*)s->Wb.p, (const float*)s->Gb.p, batch,
-                                   lr_host[b], beta1, beta2, adam_eps, step0 + b + 1,
-                                   losses_out + b, stream));
-  }
+  // each batch's rows are read through perm by the first kernels of the
+  // step (no gathered copy)
+  for (int b = 0; b < n_batches; ++b)
+    EDHMC_TRY(train_step(ctx, s, W, G, batch, lr_host[b], beta1, beta2, adam_eps, step0 + b + 1,
+                         losses_out + b, (hipStream_t)stream, perm + (int64_t)b * batch, P));
   return EDHMC_OK;
 }
 
