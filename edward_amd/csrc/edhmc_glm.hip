@@ -1241,6 +1241,10 @@ constexpr int kWsPreGrad = EDHMC_WS_PRE_GRAD;   // the same with the residual ep
 #define EDHMC_WS_PRE_GRAD1 4
 #endif
 constexpr int kWsPreGrad1 = EDHMC_WS_PRE_GRAD1; // ... on one plane without the log-likelihood
+#ifndef EDHMC_WS_WIDE1
+#define EDHMC_WS_WIDE1 1
+#endif
+constexpr bool kWsWide1 = EDHMC_WS_WIDE1 != 0;   // one-plane pass on 128-chain tiles
 constexpr int kWsMaxDPad = 512;
 #ifndef EDHMC_WS_PAD
 #define EDHMC_WS_PAD 16
@@ -1263,13 +1267,18 @@ __device__ inline void mfma16(f32x4v_t& acc, const uint4& a, const uint4& b) {
 // PLANES: 2 (hi and lo planes of W, the exact log joint) or 1 (hi plane
 // only: the one-plane inner leapfrog gradients); LL: sum y eta - softplus(eta)
 // (bern_nll_fast's non-negative form).
-template <bool WR, int PLANES, bool LL, int PRE>
+template <bool WR, int PLANES, bool LL, int PRE, int NI = 4>
 __global__ void __launch_bounds__(kWsThreads) __attribute__((amdgpu_waves_per_eu(1, 2)))
 gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
                   const bf16_t* __restrict__ X, int64_t N, int64_t N_pad, int64_t rows_per_part,
                   int parts, int mt_per_xcd, int part_per_xcd, double* __restrict__ llpart,
                   int64_t ll_ld, const float* __restrict__ y, bf16_t* __restrict__ Rt,
                   int64_t ldr) {
+  // NI chain blocks of 16 per workgroup (BMC chains; 8 only for the one-plane
+  // pass, whose single plane leaves room in LDS for 128 chains: each X
+  // fragment then feeds twice the MFMAs)
+  constexpr int BMC = 16 * NI;
+  static_assert(PLANES * BMC <= 2 * kWsBM && (!LL || NI == 4), "chain tile");
   __shared__ __align__(16) char lds[2 * kWsBM * kWsRowBytes];
   // Consecutive workgroup ids go to the 8 XCDs in turn.  With part_per_xcd
   // (parts divides 8) every XCD streams one row range for 8 / parts slices
@@ -1287,12 +1296,12 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
   if (part >= parts) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int rsb = 2 * D_pad + kWsPad;
-  // resident W: LDS row p * 64 + c = plane p of chain 64 tm + c
+  // resident W: LDS row p * BMC + c = plane p of chain BMC tm + c
   const int cpr = D_pad / 8;                 // 16-byte chunks per row
-  for (int q = tid; q < PLANES * kWsBM * cpr; q += kWsThreads) {
+  for (int q = tid; q < PLANES * BMC * cpr; q += kWsThreads) {
     const int row = q / cpr, ch = q - row * cpr;
-    const int pl = row / kWsBM, c = row - pl * kWsBM;
-    const uint4 v = *(const uint4*)(Wp + pl * plane_elems + (int64_t)(tm * kWsBM + c) * D_pad +
+    const int pl = row / BMC, c = row - pl * BMC;
+    const uint4 v = *(const uint4*)(Wp + pl * plane_elems + (int64_t)(tm * BMC + c) * D_pad +
                                     ch * 8);
     *(uint4*)(lds + row * rsb + ch * 16) = v;
   }
@@ -1320,9 +1329,9 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
   // one chain are consecutive rows, stored as one 8-byte piece -- and
   // lls[i][0] sums chain 16 i + l16.  (The swapped layout costs the forward
   // 6 %; the residual stores gain more.)
-  double lls[4][4];
+  double lls[NI][4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < NI; ++i)
 #pragma unroll
     for (int v = 0; v < 4; ++v) lls[i][v] = 0.0;
   if (n_tiles > 0) {
@@ -1336,9 +1345,9 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
 #pragma unroll
         for (int j = 0; j < 4; ++j) xr[u][j] = *(const uint4*)(rp[j] + u * 1024);
     }
-    f32x4v_t acc[4][4];
+    f32x4v_t acc[NI][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < NI; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = f32x4v_t{0.0f, 0.0f, 0.0f, 0.0f};
     // y of the wave's rows in the tile computed now (tile 0 first), lane l
@@ -1378,12 +1387,12 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
         const int ks = ks0 + u;
 #pragma unroll
         for (int pl = 0; pl < PLANES; ++pl) {
-          uint4 wf[4];
+          uint4 wf[NI];
 #pragma unroll
-          for (int i = 0; i < 4; ++i)
-            wf[i] = *(const uint4*)(lds + (pl * kWsBM + i * 16 + l16) * rsb + ks * 64 + lq);
+          for (int i = 0; i < NI; ++i)
+            wf[i] = *(const uint4*)(lds + (pl * BMC + i * 16 + l16) * rsb + ks * 64 + lq);
 #pragma unroll
-          for (int i = 0; i < 4; ++i)
+          for (int i = 0; i < NI; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
               if (WR) mfma16(acc[i][j], xr[u][j], wf[i]);
@@ -1428,7 +1437,7 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
           const int64_t rb = n_begin + t * kWsTile + wave * kWsWaveRows + 4 * (lane >> 4);
           const int64_t row0 = n_begin + t * kWsTile;
           bf16_t* rt = Rt + (row0 >> 5) * ldr * 32;
-          const uint32_t c0 = (uint32_t)(tm * kWsBM + l16) * 32u;
+          const uint32_t c0 = (uint32_t)(tm * BMC + l16) * 32u;
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             const int64_t r0 = rb + 16 * j;
@@ -1439,7 +1448,7 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
             const int nl = (int)(r0 - row0);                       // row within the tile
             const uint32_t ro = (uint32_t)(nl >> 5) * (uint32_t)ldr * 32u + (uint32_t)(nl & 31) + c0;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < NI; ++i) {
               float s32 = 0.0f;
               float rv[4];
 #pragma unroll
@@ -1469,7 +1478,7 @@ gemm_ll_ws_kernel(const bf16_t* __restrict__ Wp, int64_t plane_elems, int D_pad,
           }
         }
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < NI; ++i)
 #pragma unroll
           for (int j = 0; j < 4; ++j) acc[i][j] = f32x4v_t{0.0f, 0.0f, 0.0f, 0.0f};
       }
@@ -1744,7 +1753,10 @@ static bool ws_forward_ok(const edhmc_ctx* ctx) {
 }
 
 static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st, bool wr, int planes, bool ll) {
-  const int m_tiles = ctx->C_pad / kWsBM;
+  // the one-plane pass on 128-chain tiles where the chain tiles still split
+  // evenly over the XCDs (each X fragment feeds 8 chain blocks, not 4)
+  const bool wide = wr && planes == 1 && !ll && kWsWide1 && ctx->C_pad % (8 * 2 * kWsBM) == 0;
+  const int m_tiles = ctx->C_pad / (wide ? 2 * kWsBM : kWsBM);
   // about one workgroup per CU: split the rows of each chain tile into parts
   int parts = std::max(1, 256 / m_tiles);
   int64_t rows_per_part = (ctx->N + parts - 1) / parts;
@@ -1761,7 +1773,14 @@ static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st, bool wr, int planes
   const int grid = m_tiles * parts;
   {
     ProfScope ps(ctx, wr ? PK_GEMM_GRAD : PK_GEMM_LL, st);
-    if (wr && planes == 1 && !ll)
+    if (wide)
+      hipLaunchKernelGGL((gemm_ll_ws_kernel<true, 1, false, kWsPreGrad1, 8>), dim3(grid),
+                         dim3(kWsThreads), 0, st, (const bf16_t*)ctx->z_lp.p,
+                         (int64_t)ctx->C_pad * ctx->D_pad, ctx->D_pad, (const bf16_t*)ctx->Xf.p,
+                         ctx->N, ctx->N_pad, rows_per_part, parts, mt_per_xcd,
+                         part_per_xcd ? 1 : 0, (double*)ctx->llpart.p, (int64_t)ctx->C_pad,
+                         (const float*)ctx->y.p, (bf16_t*)ctx->Rt.p, (int64_t)ctx->C_pad);
+    else if (wr && planes == 1 && !ll)
       hipLaunchKernelGGL((gemm_ll_ws_kernel<true, 1, false, kWsPreGrad1>), dim3(grid),
                          dim3(kWsThreads), 0, st, (const bf16_t*)ctx->z_lp.p,
                          (int64_t)ctx->C_pad * ctx->D_pad, ctx->D_pad, (const bf16_t*)ctx->Xf.p,

@@ -398,6 +398,7 @@ def test_plain_bf16_transition_on_the_chain_stationary_path():
 
 @pytest.mark.parametrize('N,D,C,path', [(20000, 128, 512, 'w-stationary'),
                                         (20000, 512, 512, 'w-stationary'),
+                                        (20000, 256, 1024, 'w-stationary, 128-chain tiles'),
                                         (20000, 640, 256, 'lds-staged'),
                                         (6000, 96, 130, 'lds-staged')])
 def test_plain_bf16_transition_follows_the_rounded_oracle(N, D, C, path):
@@ -405,7 +406,8 @@ def test_plain_bf16_transition_follows_the_rounded_oracle(N, D, C, path):
   pass's own roundings (GLM.grad_bf16: hi + lo position planes at the ends,
   the hi plane for the inner steps, bf16 residuals), on the W-stationary
   kernels (bf16, D_pad <= 512, enough rows; D = 128 and 512, the latter in
-  20 row parts with a ragged last tile) and on the LDS-staged GEMMs
+  20 row parts with a ragged last tile; at 1024 chains the one-plane inner
+  gradients on 128-chain tiles) and on the LDS-staged GEMMs
   (D_pad = 640 > 512; N below 16 row tiles): two runs bitwise equal; end
   positions of every chain: median within 1e-5 relative (measured 1e-7 to
   7e-7), every chain within 1e-3 (measured up to 1.6e-4 on isolated chains
