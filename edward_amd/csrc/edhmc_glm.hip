@@ -1787,6 +1787,13 @@ static int launch_ws_forward(edhmc_ctx* ctx, hipStream_t st, bool wr, int planes
                          ctx->N, ctx->N_pad, rows_per_part, parts, mt_per_xcd,
                          part_per_xcd ? 1 : 0, (double*)ctx->llpart.p, (int64_t)ctx->C_pad,
                          (const float*)ctx->y.p, (bf16_t*)ctx->Rt.p, (int64_t)ctx->C_pad);
+    else if (wr && planes == 2 && !ll)   // exact inner gradients: no log-likelihood terms
+      hipLaunchKernelGGL((gemm_ll_ws_kernel<true, 2, false, kWsPreGrad>), dim3(grid),
+                         dim3(kWsThreads), 0, st, (const bf16_t*)ctx->z_lp.p,
+                         (int64_t)ctx->C_pad * ctx->D_pad, ctx->D_pad, (const bf16_t*)ctx->Xf.p,
+                         ctx->N, ctx->N_pad, rows_per_part, parts, mt_per_xcd,
+                         part_per_xcd ? 1 : 0, (double*)ctx->llpart.p, (int64_t)ctx->C_pad,
+                         (const float*)ctx->y.p, (bf16_t*)ctx->Rt.p, (int64_t)ctx->C_pad);
     else if (wr && planes == 2)
       hipLaunchKernelGGL((gemm_ll_ws_kernel<true, 2, true, kWsPreGrad>), dim3(grid),
                          dim3(kWsThreads), 0, st, (const bf16_t*)ctx->z_lp.p,
