@@ -91,12 +91,14 @@ def parse():
                   help='1: the burn-in step-size centre follows the acceptance rate')
   ap.add_argument('--plain-steps', type=int, default=16, help='timed plain fp32 transitions')
   ap.add_argument('--workload', default='large',
-                  choices=['large', 'hier', 'copy', 'forward', 'surrogate'],
+                  choices=['large', 'hier', 'copy', 'forward', 'gradient', 'surrogate'],
                   help="'hier': the hierarchical logistic regression (G=2048 groups of "
                        "1+Poisson(400) rows, D=64, 512 chains, L=16), its own JSON line; "
                        "'copy': 1 GiB device-to-device copies, the byte-counter calibration "
                        "of HBM traffic profiles; 'forward': --steps exact forward passes "
                        "(the gemm_ll kernel alone) on the large workload, for profiling; "
+                       "'gradient': --steps plain bf16 transitions of 4 leapfrog steps (the "
+                       "gradient-pass kernels); "
                        "'surrogate': --steps surrogate transitions after a short warm start, "
                        "for profiling")
   ap.add_argument('--hidden', type=int, default=1024)
@@ -515,6 +517,42 @@ def bench_forward(args):
                     'achieved_tflops': flop / (ms / max(n, 1) * 1e-3) / 1e12}))
 
 
+def bench_gradient(args):
+  """--steps plain bf16 HMC transitions of 4 leapfrog steps (the warm-up's
+  descent transitions: three one-plane inner gradient passes and one exact
+  two-plane pass with the log-likelihood, each GEMM1 with the residual
+  epilogue plus X^T R) of 4096 chains near w* on the large workload, for
+  counter profiles of the gradient-pass kernels."""
+  import torch
+  from edward_amd import workloads as W
+  from edward_amd.engine import GLMEngine
+  wl = W.logreg_large()
+  rng = np.random.default_rng(args.seed)
+  Wp = torch.tensor((wl.w_true[None, :] + 0.003 * rng.normal(size=(wl.C, wl.D)))
+                    .astype(np.float32), device='cuda')
+  eng = GLMEngine(wl.X, wl.y, wl.prior_loc, wl.prior_scale, precision='bf16')
+  eng.set_state(Wp)
+  L = 4
+  eng.transition(wl.step_size, L, seed=args.seed)
+  torch.cuda.synchronize()
+  eng.profile(True)
+  t0 = time.time()
+  for k in range(args.steps):
+    eng.transition(wl.step_size, L, seed=args.seed, counter=k + 1)
+  torch.cuda.synchronize()
+  secs = time.time() - t0
+  pr = eng.profile_read()
+  flop = 2.0 * wl.N * wl.D * wl.C
+  g1_ms, g1_n = pr['gemm_grad']
+  g2_ms, g2_n = pr['gemm_xtr']
+  print(json.dumps({'metric': 'plain bf16 HMC transitions of L=%d, N=%d D=%d chains=%d'
+                              % (L, wl.N, wl.D, wl.C),
+                    'value': args.steps / secs, 'unit': 'transitions/s', 'steps': args.steps,
+                    'gemm_grad_ms_per_transition': g1_ms / args.steps,
+                    'gemm_xtr_ms': g2_ms / max(g2_n, 1),
+                    'gemm_xtr_tflops': flop / (g2_ms / max(g2_n, 1) * 1e-3) / 1e12}))
+
+
 def bench_surrogate(args):
   """--steps surrogate transitions on the large workload after a short
   Laplace warm start and two training epochs: the sampling window's kernels
@@ -570,6 +608,8 @@ def main():
     return bench_forward(args)
   if args.workload == 'surrogate':
     return bench_surrogate(args)
+  if args.workload == 'gradient':
+    return bench_gradient(args)
   import torch
   from edward_amd import workloads as Wk
   from edward_amd.engine import GLMEngine, jittered_step
