@@ -764,7 +764,8 @@ static int batch_ok(edhmc_ctx* ctx, Surrogate* s, int B) {
 // the transposed activations the weight-gradient GEMMs read (their K is the
 // batch); h2t holds [h2 | x0]^T, its last Dp rows being x0^T.
 static int loss_grad(edhmc_ctx* ctx, Surrogate* s, const float* W, const float* G, int B,
-                     double* loss, hipStream_t st, const int64_t* idx = nullptr, int64_t P_rows = 0) {
+                     double* loss, hipStream_t st, const int64_t* idx = nullptr, int64_t P_rows = 0,
+                     bool backward = true) {
   const int Bp = (int)round_up(B, 128);
   const float* P = (const float*)s->P.p;
   const float* nrm = (const float*)s->nrm.p;
@@ -792,6 +793,7 @@ static int loss_grad(edhmc_ctx* ctx, Surrogate* s, const float* W, const float* 
   el.ldt = Bp;
   el.loss = loss;
   EDHMC_TRY(mlp_gemm(ctx, s->h2.p, s->Kc, s->W3b.p, s->Kc, Bp, s->Dp, s->Kc, el, st));
+  if (!backward) return EDHMC_OK;   // the loss alone
   float* Gr = (float*)s->Gr.p;
   // layer 3 weights: dW3 = dO^T [h2 | x0], db3 = column sums of dO
   EpiStoreF32 ew;
@@ -1235,7 +1237,8 @@ int edhmc_sur_loss_grad(edhmc_ctx* ctx, const float* W, const float* G, int32_t 
   if (!W || !G) return set_err(ctx, EDHMC_EINVAL, "NULL argument");
   hipStream_t st = (hipStream_t)stream;
   s->ghat_valid = false;
-  EDHMC_TRY(loss_grad(ctx, s, W, G, B, loss_out ? loss_out : (double*)s->loss.p, st));
+  EDHMC_TRY(loss_grad(ctx, s, W, G, B, loss_out ? loss_out : (double*)s->loss.p, st, nullptr, 0,
+                       param_grad_out != nullptr));
   if (param_grad_out) {
     hipLaunchKernelGGL(pack_kernel, dim3(blocks_for(s->n_params)), dim3(256), 0, st, layout_of(s),
                        s->n_params, param_grad_out, (float*)s->Gr.p, 0);

@@ -168,9 +168,10 @@ class Surrogate(object):
     self.step_count += int(n_batches)
     return losses
 
-  def validation_loss(self, W, G):
-    """Mean loss over (W, G) in batches of max_batch (no parameter update;
-    the batches' losses land in one device array, read once)."""
+  def _launch_validation(self, W, G):
+    """Launch the loss of (W, G) in batches of max_batch (forward only, no
+    parameter update); returns the batches' losses (device fp64, not yet
+    read) and their sizes."""
     eng = self.engine
     W = W.to(eng.device, torch.float32).contiguous()
     G = G.to(eng.device, torch.float32).contiguous()
@@ -183,10 +184,20 @@ class Surrogate(object):
           eng.h, L.ptr(W[s:s + n]), L.ptr(G[s:s + n]), n, L.ptr(losses[k:k + 1]), None,
           eng.stream))
       sizes.append(n)
+    return losses[:len(sizes)], sizes
+
+  @staticmethod
+  def _mean_loss(batch_losses, sizes):
     if not sizes:
       return 0.0
-    ls = losses.cpu().numpy()[:len(sizes)]
-    return float(np.dot(ls, np.asarray(sizes, np.float64)) / sum(sizes))
+    return float(np.dot(np.asarray(batch_losses, np.float64), np.asarray(sizes, np.float64))
+                 / sum(sizes))
+
+  def validation_loss(self, W, G):
+    """Mean loss over (W, G) in batches of max_batch (no parameter update;
+    the batches' losses land in one device array, read once)."""
+    losses, sizes = self._launch_validation(W, G)
+    return self._mean_loss(losses.cpu().numpy(), sizes)
 
   def fit(self, W, G, epochs=20, batch=4096, lr=1e-3, lr_final=1e-4, seed=0, normalise=True,
           log=None, val_frac=0.0):
@@ -228,16 +239,24 @@ class Surrogate(object):
     for ep in range(epochs):
       lrs = [lr * decay ** (step + b) for b in range(n_batches)]
       ep_losses = self.train_epoch(W, G, perm, batch, lrs)
-      # the next epoch's shuffle is drawn while this one runs on the device
-      # (the copy waits for the epoch's use of the current one: same stream)
+      # the held-out loss is queued behind the epoch, and the next epoch's
+      # shuffle drawn and sent (from pinned memory, without waiting) while
+      # they run; the host then reads both losses at once
+      if best is not None:
+        vl_dev, vsizes = self._launch_validation(Wv, Gv)
+        read = torch.cat([ep_losses, vl_dev])
+      else:
+        read = ep_losses
       if ep + 1 < epochs:
-        perm = torch.as_tensor(train_rows[rng.permutation(P)], dtype=torch.int64).to(dev)
-      ep_loss = float(ep_losses.cpu().numpy().mean())
+        perm = torch.as_tensor(train_rows[rng.permutation(P)], dtype=torch.int64).pin_memory()
+        perm = perm.to(dev, non_blocking=True)
+      vals = read.cpu().numpy()
+      ep_loss = float(vals[:n_batches].mean())
       step += n_batches
       losses.append(ep_loss)
       msg = 'surrogate epoch %d loss %.5f' % (ep, ep_loss)
       if best is not None:
-        vl = self.validation_loss(Wv, Gv)
+        vl = self._mean_loss(vals[n_batches:], vsizes)
         msg += ' held-out %.5f' % vl
         if vl < best[0]:
           best = (vl, self.get_params().clone())

@@ -355,7 +355,9 @@ int edhmc_sur_forward(edhmc_ctx* ctx, const float* W, int32_t B, float* grad_out
 /* Loss = mean over batch and coordinates of (out - target_n)^2 with
  * target_n = (G - out_shift) / out_scale.  Writes the loss (fp64, device
  * [1]) and the gradient of the loss w.r.t. the flat parameters (fp32, device,
- * edhmc_sur_num_params entries).  Parity hook for the backward kernels. */
+ * edhmc_sur_num_params entries); with param_grad_out NULL only the loss is
+ * computed (no backward pass: held-out losses).  Parity hook for the
+ * backward kernels. */
 int edhmc_sur_loss_grad(edhmc_ctx* ctx, const float* W, const float* G, int32_t B,
                         double* loss_out, float* param_grad_out, void* stream);
 /* One Adam step (Kingma & Ba 2015) on a batch: loss_grad then
