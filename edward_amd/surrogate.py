@@ -205,8 +205,9 @@ class Surrogate(object):
     every epoch, learning rate decayed geometrically from lr to lr_final.
     With val_frac > 0 that fraction of the pairs is held out and the weights
     of the epoch with the lowest held-out loss are kept (early stopping).
-    Shuffles are host permutations of row indices; the batches are gathered
-    on the device (``edhmc_sur_train_epoch``, ``edhmc_gather_rows``).
+    Shuffles are host permutations of row indices; each epoch's batches
+    are read through them by ``edhmc_sur_train_epoch``, the held-out rows
+    gathered once (``edhmc_gather_rows``).
     Returns the mean training loss of each epoch."""
     eng = self.engine
     dev = eng.device

@@ -370,7 +370,7 @@ int edhmc_sur_train_step(edhmc_ctx* ctx, const float* W, const float* G, int32_t
  * G[perm[b*batch+i]] (i < batch; W, G [P, D], perm int64 device), trained
  * as edhmc_sur_train_step with learning rate lr_host[b] (host array) and
  * Adam step step0 + b + 1; losses_out [n_batches] fp64 (device).
- * Permutation entries outside [0, P) gather NaN rows (the batch's loss is
+ * Permutation entries outside [0, P) read as NaN rows (the batch's loss is
  * NaN).  Equal to the sequence of single steps, without a host round trip
  * per batch. */
 int edhmc_sur_train_epoch(edhmc_ctx* ctx, const float* W, const float* G, int64_t P,
