@@ -48,14 +48,16 @@ struct Profiler {
   }
 };
 
-// A device allocation owned by the context.
+// A device allocation owned by the context, or (view) a part of another one.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool view = false;
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && !view) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    view = false;
   }
 };
 

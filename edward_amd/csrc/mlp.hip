@@ -80,6 +80,7 @@ struct Surrogate {
   // other one, so no GEMM operand changes while the GEMM runs
   DevBuf h2x;
   DevBuf dO, da2, da1, h1t, h2t, dOt, da2t, da1t;
+  DevBuf pool;   // one allocation: P .. loss are views into it (edhmc_sur_create)
   DevBuf loss;                         // fp64 [1]
   // sampling state
   DevBuf ghat, ghat0;                  // [C x D_pad of the GLM]
@@ -96,7 +97,7 @@ void surrogate_destroy(Surrogate* s) {
   DevBuf* b[] = {&s->P, &s->Gr, &s->Am, &s->Av, &s->W1b, &s->W2b, &s->W3b, &s->W2t, &s->W3t,
                  &s->nrm, &s->x0, &s->a1, &s->h1, &s->a2, &s->h2, &s->h2x, &s->o, &s->dO, &s->da2,
                  &s->da1, &s->h1t, &s->h2t, &s->dOt, &s->da2t, &s->da1t, &s->loss,
-                 &s->ghat, &s->ghat0};
+                 &s->ghat, &s->ghat0, &s->pool};
   for (DevBuf* x : b) x->release();
   delete s;
 }
@@ -1114,33 +1115,29 @@ int edhmc_sur_create(edhmc_ctx* ctx, int32_t D, int32_t H, int32_t max_batch) {
   s->ob3 = s->oW3 + (int64_t)s->Dp * s->Kc;
   s->n_pad = s->ob3 + s->Dp;
   const size_t pb = (size_t)s->n_pad * 4;
-  EDHMC_TRY(dev_alloc(ctx, s->P, pb));
-  EDHMC_TRY(dev_alloc(ctx, s->Gr, pb));
-  EDHMC_TRY(dev_alloc(ctx, s->Am, pb));
-  EDHMC_TRY(dev_alloc(ctx, s->Av, pb));
   const size_t hd = (size_t)s->Hp * s->Dp * 2, hh = (size_t)s->Hp * s->Hp * 2;
-  EDHMC_TRY(dev_alloc(ctx, s->W1b, hd));
-  EDHMC_TRY(dev_alloc(ctx, s->W2b, hh));
-  EDHMC_TRY(dev_alloc(ctx, s->W3b, (size_t)s->Dp * s->Kc * 2));
-  EDHMC_TRY(dev_alloc(ctx, s->W2t, hh));
-  EDHMC_TRY(dev_alloc(ctx, s->W3t, (size_t)s->Dp * s->Kc * 2));
-  EDHMC_TRY(dev_alloc(ctx, s->nrm, (size_t)4 * s->Dp * 4));
+  const size_t dk = (size_t)s->Dp * s->Kc * 2, bk = (size_t)s->Bp * s->Kc * 2;
   const size_t bd = (size_t)s->Bp * s->Dp, bh = (size_t)s->Bp * s->Hp;
-  EDHMC_TRY(dev_alloc(ctx, s->x0, bd * 2));
-  EDHMC_TRY(dev_alloc(ctx, s->a1, bh * 4));
-  EDHMC_TRY(dev_alloc(ctx, s->h1, bh * 2));
-  EDHMC_TRY(dev_alloc(ctx, s->a2, bh * 4));
-  EDHMC_TRY(dev_alloc(ctx, s->h2, (size_t)s->Bp * s->Kc * 2));
-  EDHMC_TRY(dev_alloc(ctx, s->h2x, (size_t)s->Bp * s->Kc * 2));
-  EDHMC_TRY(dev_alloc(ctx, s->dO, bd * 2));
-  EDHMC_TRY(dev_alloc(ctx, s->da2, bh * 2));
-  EDHMC_TRY(dev_alloc(ctx, s->da1, bh * 2));
-  EDHMC_TRY(dev_alloc(ctx, s->h1t, bh * 2));
-  EDHMC_TRY(dev_alloc(ctx, s->h2t, (size_t)s->Bp * s->Kc * 2));
-  EDHMC_TRY(dev_alloc(ctx, s->dOt, bd * 2));
-  EDHMC_TRY(dev_alloc(ctx, s->da2t, bh * 2));
-  EDHMC_TRY(dev_alloc(ctx, s->da1t, bh * 2));
-  EDHMC_TRY(dev_alloc(ctx, s->loss, 8));
+  // every buffer a view into one zeroed allocation (one hipMalloc and one
+  // hipMemset instead of one of each per buffer)
+  const std::pair<DevBuf*, size_t> parts[] = {
+      {&s->P, pb}, {&s->Gr, pb}, {&s->Am, pb}, {&s->Av, pb},
+      {&s->W1b, hd}, {&s->W2b, hh}, {&s->W3b, dk}, {&s->W2t, hh}, {&s->W3t, dk},
+      {&s->nrm, (size_t)4 * s->Dp * 4},
+      {&s->x0, bd * 2}, {&s->a1, bh * 4}, {&s->h1, bh * 2}, {&s->a2, bh * 4},
+      {&s->h2, bk}, {&s->h2x, bk}, {&s->dO, bd * 2}, {&s->da2, bh * 2}, {&s->da1, bh * 2},
+      {&s->h1t, bh * 2}, {&s->h2t, bk}, {&s->dOt, bd * 2}, {&s->da2t, bh * 2},
+      {&s->da1t, bh * 2}, {&s->loss, 8}};
+  size_t total = 0;
+  for (const auto& pt : parts) total += (size_t)round_up((int64_t)pt.second, 256);
+  EDHMC_TRY(dev_alloc(ctx, s->pool, total));
+  char* base = (char*)s->pool.p;
+  for (const auto& pt : parts) {
+    pt.first->p = base;
+    pt.first->bytes = pt.second;
+    pt.first->view = true;
+    base += round_up((int64_t)pt.second, 256);
+  }
   return EDHMC_OK;
 }
 
