@@ -34,13 +34,20 @@ class Surrogate(object):
     self.D = engine.D
     self.H = int(hidden)
     self.max_batch = int(max_batch)
+    # the initial weights are drawn on the host before the library call,
+    # whose allocation waits for the device: work queued before runs meanwhile
+    flat = self._initial_params(seed)
     engine.check(engine.lib.edhmc_sur_create(engine.h, self.D, self.H, self.max_batch))
     self.n_params = int(engine.lib.edhmc_sur_num_params(engine.h))
+    self.set_params(flat)
     self.step_count = 0
-    self.init_params(seed)
 
   # -- parameters ----------------------------------------------------------
   def init_params(self, seed=0):
+    self.set_params(self._initial_params(seed))
+    self.step_count = 0
+
+  def _initial_params(self, seed):
     rng = np.random.default_rng(seed)
     D, H = self.D, self.H
 
@@ -50,10 +57,8 @@ class Surrogate(object):
     # output layer [hidden | skip]: small hidden part, zero skip (fit() starts
     # the skip at the least-squares linear map)
     W3 = np.concatenate([0.1 * glorot(D, H), np.zeros((D, D))], axis=1)
-    flat = np.concatenate([glorot(H, D).ravel(), np.zeros(H), glorot(H, H).ravel(), np.zeros(H),
+    return np.concatenate([glorot(H, D).ravel(), np.zeros(H), glorot(H, H).ravel(), np.zeros(H),
                            W3.ravel(), np.zeros(D)]).astype(np.float32)
-    self.set_params(flat)
-    self.step_count = 0
 
   def _offsets(self):
     D, H = self.D, self.H
@@ -83,16 +88,21 @@ class Surrogate(object):
     xtx = xtx.cpu().numpy()
     xtt = xtt.cpu().numpy()
     A = np.linalg.solve(xtx + ridge * len(sel) * np.eye(self.D), xtt)   # t ~ x A
-    flat = self.get_params().cpu().numpy()
+    # written into the device copy of the parameters (only the D x D block
+    # crosses to the device)
+    flat = self.get_params()
     o = self._offsets()
-    W3 = flat[o['W3']:o['b3']].reshape(self.D, self.H + self.D)
-    W3[:, self.H:] = A.T
+    W3 = flat[o['W3']:o['b3']].view(self.D, self.H + self.D)
+    W3[:, self.H:] = torch.from_numpy(np.ascontiguousarray(A.T, dtype=np.float32)).to(eng.device)
     flat[o['b3']:o['b3'] + self.D] = 0.0
     self.set_params(flat)
     self.step_count = 0
 
   def set_params(self, flat):
-    p = torch.as_tensor(np.asarray(flat, np.float32)).to(self.engine.device)
+    if isinstance(flat, torch.Tensor):
+      p = flat.to(self.engine.device, torch.float32).contiguous()
+    else:
+      p = torch.as_tensor(np.asarray(flat, np.float32)).to(self.engine.device)
     if p.numel() != self.n_params:
       raise ValueError('expected %d parameters, got %d' % (self.n_params, p.numel()))
     self.engine.check(self.engine.lib.edhmc_sur_set_params(self.engine.h, L.ptr(p),
@@ -265,7 +275,7 @@ class Surrogate(object):
         log(msg)
     if best is not None:
       steps = self.step_count
-      self.set_params(best[1].cpu().numpy())
+      self.set_params(best[1])
       self.step_count = steps
       self.best_validation_loss = best[0]
     return losses
