@@ -61,6 +61,10 @@ def test_loss_grad_and_forward_match_oracle(D, H, B):
   W = rng.normal(size=(B, D)).astype(np.float32)
   G = rng.normal(size=(B, D)).astype(np.float32) * 3
   loss, grad = sur.loss_grad(torch.tensor(W), torch.tensor(G))
+  # the loss alone (forward only, as for held-out batches): the same value up
+  # to the order of its atomically added FP64 partials
+  assert sur.validation_loss(torch.tensor(W), torch.tensor(G)) == pytest.approx(loss.item(),
+                                                                                rel=1e-12)
   p = MO.unflatten(sur.get_params().cpu().numpy(), D, H)
   loss_ref, g_ref = MO.loss_and_grad(p, W.astype(np.float64), G.astype(np.float64), norm)
   assert loss.item() == pytest.approx(loss_ref, rel=3e-2)
@@ -130,8 +134,8 @@ def test_adam_training_matches_oracle_step_by_step():
 
 
 def test_train_epoch_equals_single_steps():
-  """One edhmc_sur_train_epoch call (batches gathered by a permutation on the
-  device, per-step learning rates) leaves the same weights, bit for bit, as
+  """One edhmc_sur_train_epoch call (batch rows read through a permutation on
+  the device, per-step learning rates) leaves the same weights, bit for bit, as
   the same steps taken one by one on index_select batches, and the same
   losses up to the last bit (the loss adds per-wave FP64 partials atomically,
   in whatever order the waves finish); the oracle pins the single steps (the
@@ -161,7 +165,7 @@ def test_train_epoch_equals_single_steps():
   np.testing.assert_allclose(la, np.array(lb), rtol=1e-12)
   np.testing.assert_array_equal(pa, b.get_params().cpu().numpy())
   assert a.step_count == b.step_count == nb
-  # an index outside the pairs gathers a NaN row: that step's loss is NaN
+  # an index outside the pairs reads a NaN row: that step's loss is NaN
   bad = perm.clone()
   bad[3] = Wtr.shape[0]
   c = _surrogate(eng, H=H, B=B, seed=11)
