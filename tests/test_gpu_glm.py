@@ -245,15 +245,16 @@ def test_run_semantics_and_moments():
   assert n_acc.sum().item() > 0
 
 
-@pytest.mark.parametrize('N,D,C', [(65536, 512, 512), (100000, 256, 256)])
+@pytest.mark.parametrize('N,D,C', [(65536, 512, 512), (100000, 256, 256), (65536, 768, 256)])
 def test_bf16_gradient_on_the_direct_load_gemm2(N, D, C):
   """bf16 gradients at shapes whose X^T R product runs on the 256 x 256
   direct-to-LDS kernel (whole tiles, every K split >= 5 stages): 64 splits
   of 32 stages (a whole number of five-stage groups plus a partial one),
-  and 64 splits of 49 stages with a last split of 41 (stages past a split's
-  end read as zeros), against the oracle's gradient with the pass's own
+  64 splits of 49 stages with a last split of 41 (stages past a split's end
+  read as zeros), and three column tiles at D = 768 (GEMM1 there on the
+  LDS-staged kernel), against the oracle's gradient with the pass's own
   roundings (GLM.grad_bf16, two position planes, bf16 residuals, FP64
-  sums): max error within 1e-4 of max|grad| (measured 2.7e-5 and 1.4e-5:
+  sums): max error within 1e-4 of max|grad| (measured 2.7e-5, 1.4e-5, 3.4e-5:
   fp32 sums over up to 100,000 rows, and residuals whose fp32 eta rounds to
   the other side of a bf16 tie), two evaluations bitwise equal."""
   from edward_amd import workloads as W
