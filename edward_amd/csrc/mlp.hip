@@ -80,8 +80,9 @@ struct Surrogate {
   // other one, so no GEMM operand changes while the GEMM runs
   DevBuf h2x;
   DevBuf dO, da2, da1, h1t, h2t, dOt, da2t, da1t;
-  DevBuf pool;   // one allocation: P .. loss are views into it (edhmc_sur_create)
+  DevBuf pool;   // one allocation: P .. lpart are views into it (edhmc_sur_create)
   DevBuf loss;                         // fp64 [1]
+  DevBuf lpart;                        // fp64 [Bp / 32 x Dp / 32] loss partials (EpiLoss)
   // sampling state
   DevBuf ghat, ghat0;                  // [C x D_pad of the GLM]
   bool ghat_valid = false;
@@ -97,7 +98,7 @@ void surrogate_destroy(Surrogate* s) {
   DevBuf* b[] = {&s->P, &s->Gr, &s->Am, &s->Av, &s->W1b, &s->W2b, &s->W3b, &s->W2t, &s->W3t,
                  &s->nrm, &s->x0, &s->a1, &s->h1, &s->a2, &s->h2, &s->h2x, &s->o, &s->dO, &s->da2,
                  &s->da1, &s->h1t, &s->h2t, &s->dOt, &s->da2t, &s->da1t, &s->loss,
-                 &s->ghat, &s->ghat0, &s->pool};
+                 &s->lpart, &s->ghat, &s->ghat0, &s->pool};
   for (DevBuf* x : b) x->release();
   delete s;
 }
@@ -194,7 +195,11 @@ struct EpiLoss {
   bf16_t* dO;
   bf16_t* dOt;               // dO transposed [Dp x ldt]
   int64_t ldt;
-  double* loss;              // fp64 accumulator
+  // per-wave FP64 loss partials, one slot per 32 x 32 block [Bp / 32 x
+  // lcols] (a wave writes the slot of its first block; the rest stay 0),
+  // summed in a fixed order by loss_sum_kernel
+  double* lpart;
+  int lcols;                 // Dp / 32
   template <typename Cfg>
   __device__ void apply(f32x16_t (&acc)[Cfg::kWM][Cfg::kWN], const EpiPos& p) const {
     double part = 0.0;
@@ -232,7 +237,7 @@ struct EpiLoss {
         }
     }
     part = wave_sum_f64(part) * (0.5 * (double)coef);   // mean over B x D
-    if (p.lane == 0) atomicAdd(loss, part);
+    if (p.lane == 0) lpart[(int64_t)(p.m0 >> 5) * lcols + (p.n0 >> 5)] = part;
   }
 };
 
@@ -388,9 +393,10 @@ __global__ void norm_in_kernel(const float* __restrict__ W, int B, int D, int64_
                                const float* __restrict__ shift, const float* __restrict__ scale,
                                bf16_t* __restrict__ x0, int Bp, int Dp, bf16_t* __restrict__ xcat,
                                int xcat_ld, double* __restrict__ zero_out = nullptr,
-                               const int64_t* __restrict__ idx = nullptr, int64_t P = 0) {
+                               int n_zero = 0, const int64_t* __restrict__ idx = nullptr,
+                               int64_t P = 0) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (zero_out && i == 0) *zero_out = 0.0;   // a loss accumulator the batch's GEMMs add to
+  if (i < n_zero) zero_out[i] = 0.0;   // the batch's loss partial slots
   if (i >= (int64_t)Bp * Dp) return;
   const int b = (int)(i / Dp), d = (int)(i - (int64_t)b * Dp);
   float v = 0.0f;
@@ -419,6 +425,24 @@ __global__ void gather_rows_kernel(const float* __restrict__ src, int64_t P, int
   const int b = (int)(i / D), d = (int)(i - (int64_t)b * D);
   const int64_t r = idx[b];
   dst[i] = (r >= 0 && r < P) ? src[r * D + d] : __builtin_nanf("");
+}
+
+// *loss = the sum of the n per-wave partials in a fixed order (256 strided
+// sums, then a fixed tree), so the loss has the same bits on every run.
+__global__ void __launch_bounds__(256) loss_sum_kernel(const double* __restrict__ part, int n,
+                                                       double* __restrict__ loss) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  double x = 0.0;
+  for (int k = t; k < n; k += 256) x += part[k];
+  red[t] = x;
+  __syncthreads();
+#pragma unroll
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) *loss = red[0];
 }
 
 // out [C x R] = in [R x C]^T for bf16, 64 x 64 tiles (R, C multiples of 64);
@@ -759,7 +783,7 @@ static int batch_ok(edhmc_ctx* ctx, Surrogate* s, int B) {
 }
 
 // Forward and backward over a batch: the parameter gradient into s->Gr, the
-// mean loss accumulated into *loss (FP64, zeroed by the first kernel).  The
+// mean loss into *loss (FP64, per-wave partials summed in a fixed order).  The
 // batch is rows idx[0..B) of W and G ([P x D]) when idx is given, else their
 // first B rows.  The forward and activation-gradient epilogues also write
 // the transposed activations the weight-gradient GEMMs read (their K is the
@@ -772,9 +796,11 @@ static int loss_grad(edhmc_ctx* ctx, Surrogate* s, const float* W, const float* 
   const float* nrm = (const float*)s->nrm.p;
   bf16_t* h2t = (bf16_t*)s->h2t.p;
   bf16_t* x0t = h2t + (int64_t)s->Hp * Bp;
+  double* lpart = (double*)s->lpart.p;
+  const int lcols = s->Dp / 32, n_slots = Bp / 32 * lcols;
   hipLaunchKernelGGL(norm_in_kernel, dim3(blocks_for((int64_t)Bp * s->Dp)), dim3(256), 0, st, W,
                      B, s->D, (int64_t)s->D, nrm, nrm + s->Dp, (bf16_t*)s->x0.p, Bp, s->Dp,
-                     (bf16_t*)s->h2.p + s->Hp, s->Kc, loss, idx, P_rows);
+                     (bf16_t*)s->h2.p + s->Hp, s->Kc, lpart, n_slots, idx, P_rows);
   EDHMC_TRY(check_launch(ctx, "norm_in"));
   EDHMC_TRY(transpose(ctx, TransposeJob{(const bf16_t*)s->x0.p, Bp, s->Dp, x0t}, nullptr, st));
   EDHMC_TRY(forward_hidden(ctx, s, Bp, true, st, nullptr, false, (bf16_t*)s->h1t.p, h2t));
@@ -792,8 +818,12 @@ static int loss_grad(edhmc_ctx* ctx, Surrogate* s, const float* W, const float* 
   el.dO = (bf16_t*)s->dO.p;
   el.dOt = (bf16_t*)s->dOt.p;
   el.ldt = Bp;
-  el.loss = loss;
+  el.lpart = lpart;
+  el.lcols = lcols;
   EDHMC_TRY(mlp_gemm(ctx, s->h2.p, s->Kc, s->W3b.p, s->Kc, Bp, s->Dp, s->Kc, el, st));
+  hipLaunchKernelGGL(loss_sum_kernel, dim3(1), dim3(256), 0, st, (const double*)lpart, n_slots,
+                     loss);
+  EDHMC_TRY(check_launch(ctx, "loss_sum"));
   if (!backward) return EDHMC_OK;   // the loss alone
   float* Gr = (float*)s->Gr.p;
   // layer 3 weights: dW3 = dO^T [h2 | x0], db3 = column sums of dO
@@ -1127,7 +1157,7 @@ int edhmc_sur_create(edhmc_ctx* ctx, int32_t D, int32_t H, int32_t max_batch) {
       {&s->x0, bd * 2}, {&s->a1, bh * 4}, {&s->h1, bh * 2}, {&s->a2, bh * 4},
       {&s->h2, bk}, {&s->h2x, bk}, {&s->dO, bd * 2}, {&s->da2, bh * 2}, {&s->da1, bh * 2},
       {&s->h1t, bh * 2}, {&s->h2t, bk}, {&s->dOt, bd * 2}, {&s->da2t, bh * 2},
-      {&s->da1t, bh * 2}, {&s->loss, 8}};
+      {&s->da1t, bh * 2}, {&s->loss, 8}, {&s->lpart, (size_t)(s->Bp / 32) * (s->Dp / 32) * 8}};
   size_t total = 0;
   for (const auto& pt : parts) total += (size_t)round_up((int64_t)pt.second, 256);
   EDHMC_TRY(dev_alloc(ctx, s->pool, total));

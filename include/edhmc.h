@@ -356,7 +356,8 @@ int edhmc_sur_forward(edhmc_ctx* ctx, const float* W, int32_t B, float* grad_out
  * target_n = (G - out_shift) / out_scale.  Writes the loss (fp64, device
  * [1]) and the gradient of the loss w.r.t. the flat parameters (fp32, device,
  * edhmc_sur_num_params entries); with param_grad_out NULL only the loss is
- * computed (no backward pass: held-out losses).  Parity hook for the
+ * computed (no backward pass: held-out losses).  The loss is a fixed-order
+ * sum of per-wave partials: the same bits on every run.  Parity hook for the
  * backward kernels. */
 int edhmc_sur_loss_grad(edhmc_ctx* ctx, const float* W, const float* G, int32_t B,
                         double* loss_out, float* param_grad_out, void* stream);
